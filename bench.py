@@ -1,0 +1,228 @@
+#!/usr/bin/env python3
+"""Benchmark of the dense 2-D filtering path on one MI355X.
+
+Workload (a "step"): imfilter(Float32, img, centered(K), "reflect",
+Algorithm.FIR()) with img a 4096x4096 Float32 image of seeded uniform [0,1)
+values and K a 15x15 kernel of seeded N(0,1) taps normalised to sum 1 -- the
+split-bf16 MFMA banded-Toeplitz kernel.  Input and output stay resident in
+HBM; K steps are timed with HIP events on the library's stream.
+
+Prints one JSON line: Gpixel/s, the roofline of the dominant kernel, the
+FP64 CPU baseline (the oracle's single-threaded direct FIR on a row sample),
+the max relative error of the timed output against FP64 on that sample, and
+extra measurements of the other configurations.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "imagefiltering.jl_amd"))
+sys.path.insert(0, REPO)
+
+HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def norm_kernel(rng, K):
+    while True:
+        k = rng.standard_normal((K, K))
+        s = k.sum()
+        if abs(s) >= 1:
+            return k / s
+
+
+class Timer:
+    def __init__(self, L, ctx):
+        self.L, self.ctx = L, ctx
+
+    def run(self, fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        self.L.ifm_ctx_synchronize(self.ctx)
+        self.L.ifm_timer_start(self.ctx)
+        for _ in range(steps):
+            fn()
+        ms = C.c_float()
+        self.L.ifm_timer_stop(self.ctx, C.byref(ms))
+        self.L.ifm_ctx_synchronize(self.ctx)
+        return ms.value / steps
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--profile-copy", action="store_true",
+                    help="also run one plain copy of a known size (rocprof counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the filtering path runs on a single MI355X "
+              "(multi-GPU sharding is a design note only); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import imagefiltering as IF
+    from imagefiltering import _lib
+    from imagefiltering.device import ifm_array
+    from imagefiltering.imfilter import _border_struct, _kernel_struct
+
+    L = _lib.load()
+    ctx = _lib.ctx()
+    T = Timer(L, ctx)
+
+    n = 4096
+    rng = np.random.default_rng(20261204)
+    img = rng.random((n, n), dtype=np.float32)
+    k15 = norm_kernel(np.random.default_rng(20261214), 15)
+    d_img = IF.DeviceArray.from_host(img)
+    d_out = IF.DeviceArray((n, n), IF.Float32)
+    a_in, a_out = ifm_array(d_img), ifm_array(d_out)
+    b_reflect = _border_struct(IF.Pad("reflect"), IF.Float32)
+    kern15 = IF.centered(k15)
+    ks15 = _kernel_struct(kern15)
+    keep15 = ks15
+
+    def step():
+        rc = L.ifm_imfilter_dense(ctx, a_in, a_out, None, keep15, b_reflect, _lib.PREC_AUTO)
+        if rc:
+            _lib.check(rc)
+
+    log("bench: dense 15x15 reflect on 4096x4096 f32 (MFMA split-bf16), warmup %d steps %d" % (args.warmup,
+                                                                                                 args.steps))
+    ms = T.run(step, args.steps, args.warmup)
+    px = n * n
+    gpix = px / (ms * 1e-3) / 1e9
+    bytes_alg = px * (4 + 4)
+    flops_alg = px * 2 * 15 * 15
+    achieved_gbs = bytes_alg / (ms * 1e-3) / 1e9
+    log("bench: %.4f ms/step, %.2f Gpix/s, %.0f GB/s" % (ms, gpix, achieved_gbs))
+
+    # accuracy of the timed output against the FP64 oracle on a row sample
+    out = d_out.to_host()
+    from oracle import oracle as O
+    rows = 64
+    r0 = n // 2 - rows // 2
+    t0 = time.time()
+    want = O.imfilter_dense_rows(img.astype(np.float64), k15, [-7, -7], "reflect", r0, 4)
+    t_small = time.time() - t0
+    nrows = int(max(4, min(n, 4 * args.cpu_seconds / max(t_small, 1e-3))))
+    r0 = max(0, n // 2 - nrows // 2)
+    nrows = min(nrows, n - r0)
+    t0 = time.time()
+    want = O.imfilter_dense_rows(img.astype(np.float64), k15, [-7, -7], "reflect", r0, nrows)
+    t_cpu = time.time() - t0
+    cpu_gpix = n * nrows / t_cpu / 1e9
+    got = out[:, r0:r0 + nrows].astype(np.float64)
+    err = float(np.max(np.abs(got - want)) / np.max(np.abs(want)))
+    log("bench: cpu baseline %.4f Gpix/s on %d rows (%.1f s); max rel err vs FP64 %.2e" % (cpu_gpix, nrows, t_cpu,
+                                                                                          err))
+
+    extras = {}
+    if not args.no_extras:
+        extras = run_extras(IF, _lib, L, ctx, T, img, d_img, d_out, b_reflect, args)
+    if args.profile_copy:
+        # a plain f32 copy of a known size: 256 MiB read + 256 MiB written
+        m = 1 << 26
+        src = IF.DeviceArray((m,), IF.Float32)
+        dst = IF.DeviceArray((m,), IF.Float32)
+        b = _lib.IfmBorder()
+        b.style = _lib.REPLICATE
+        _lib.check(L.ifm_padarray(ctx, ifm_array(src), ifm_array(dst), b))
+        L.ifm_ctx_synchronize(ctx)
+
+    traffic = None
+    tpath = os.path.join(REPO, "profiles", "traffic_dense_mfma_4096_k15.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get("bytes_per_launch")
+
+    rec = {
+        "metric": "Gpix/s & %HBM peak; served p50/p99 vs plain imfilter at rel err<=1e-5",
+        "value": round(gpix, 3),
+        "unit": "Gpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded uniform [0,1) image, seeded N(0,1) taps normalised to sum 1)",
+        "config": {"workload": "imfilter(Float32, img 4096x4096 Float32, centered(15x15 N(0,1)/sum), \"reflect\", "
+                               "Algorithm.FIR()) -- dense 2-D, MFMA split-bf16 banded-Toeplitz",
+                   "image": [n, n], "kernel": [15, 15], "border": "reflect", "out_eltype": "Float32"},
+        "max_rel_err_vs_fp64": err,
+        "roofline": {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "k_dense_mfma<8,16>",
+                     "mfma_bf16_tflops_issued": round(3 * flops_alg * (32.0 / 15.0) / (ms * 1e-3) / 1e12, 1),
+                     "algorithmic_tflops_fp32": round(flops_alg / (ms * 1e-3) / 1e12, 2)},
+        "cpu_baseline": {"value": round(cpu_gpix, 6), "unit": "Gpix/s", "cores": 1, "kind": "port",
+                         "sample": "rows %d..%d (%d x 4096 px) of the same image and kernel, FP64 direct FIR "
+                                   "(oracle/ifm_oracle.c, single thread like CPU1(Algorithm.FIR()))"
+                                   % (r0, r0 + nrows - 1, nrows)},
+        "extras": extras,
+    }
+    print(json.dumps(rec), flush=True)
+
+
+def run_extras(IF, _lib, L, ctx, T, img, d_img, d_out, b_reflect, args):
+    """The other configurations at their full sizes, a few steps each."""
+    from imagefiltering.device import ifm_array
+    from imagefiltering.imfilter import _border_struct, _kernel_struct
+    ex = {}
+    n = img.shape[0]
+    steps = max(3, args.steps // 10)
+
+    def rate(name, fn, px, bytes_px, steps=steps):
+        ms = T.run(fn, steps, 2)
+        gbs = px * bytes_px / (ms * 1e-3) / 1e9
+        ex[name] = {"ms": round(ms, 4), "gpix_s": round(px / (ms * 1e-3) / 1e9, 3), "hbm_gbs": round(gbs, 1),
+                    "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+        log("extra %-28s %8.3f ms  %8.2f Gpix/s  %7.0f GB/s" % (name, ms, ex[name]["gpix_s"], gbs))
+
+    a_in, a_out = ifm_array(d_img), ifm_array(d_out)
+    k31 = IF.centered(norm_kernel(np.random.default_rng(20261215), 31))
+    ks31 = _kernel_struct(k31)
+    rate("dense31_mfma_4096", lambda: L.ifm_imfilter_dense(ctx, a_in, a_out, None, ks31, b_reflect, 0), n * n, 8)
+    k15 = IF.centered(norm_kernel(np.random.default_rng(20261214), 15))
+    ks15 = _kernel_struct(k15)
+    rate("dense15_fft_4096", lambda: L.ifm_imfilter_fft(ctx, a_in, a_out, None, ks15, b_reflect), n * n, 8, 3)
+    rate("dense15_valu_fp32_4096", lambda: L.ifm_imfilter_dense(ctx, a_in, a_out, None, ks15, b_reflect, 1), n * n, 8, 3)
+    # BF16 storage in and out
+    xb = IF.DeviceArray.from_host(IF.bf16(img))
+    ob = IF.DeviceArray((n, n), IF.BFloat16)
+    a_b, a_ob = ifm_array(xb), ifm_array(ob)
+    ks15 = _kernel_struct(k15)
+    rate("dense15_mfma_bf16io_4096", lambda: L.ifm_imfilter_dense(ctx, a_b, a_ob, None, ks15, b_reflect, 0), n * n, 4)
+    # config 1: 1024^2 gaussian sigma 3 replicate, F32 -> F64
+    c1 = IF.DeviceArray.from_host(np.random.default_rng(1).random((1024, 1024), dtype=np.float32))
+    o1 = IF.DeviceArray((1024, 1024), IF.Float64)
+    kern = IF.KernelFactors.gaussian((3.0, 3.0))
+    fa = (_lib.IfmFactor * 2)()
+    keep = []
+    for i, f in enumerate(kern):
+        t = np.ascontiguousarray(f.data.data, dtype=np.float64)
+        keep.append(t)
+        fa[i].axis, fa[i].len, fa[i].first = f.Npre, len(t), f.data.first[0]
+        fa[i].taps = t.ctypes.data_as(C.POINTER(C.c_double))
+    a1, ao1 = ifm_array(c1), ifm_array(o1)
+    brep = _border_struct(IF.Pad("replicate"), IF.Float32)
+    rate("gauss13_sep_1024_f64out", lambda: L.ifm_imfilter_fir(ctx, a1, ao1, None, fa, 2, brep), 1024 * 1024, 12, 20)
+    return ex
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,826 @@
+// ifm_api.hip — the C ABI of libifm: argument validation (raising the
+// reference's ArgumentError / DimensionMismatch cases as status codes), layout
+// canonicalisation, host-array staging and kernel dispatch.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "ifm_internal.h"
+
+using namespace ifm;
+
+struct ifm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::vector<std::pair<void *, size_t>> slots;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // Device copies of dense kernels keyed by a hash of their FP32 taps, so a
+  // kernel used call after call is uploaded once (no per-call copy or sync).
+  struct TapEntry {
+    uint64_t hash;
+    size_t n;
+    void *dev;
+  };
+  std::vector<TapEntry> taps;
+};
+
+namespace {
+
+enum Slot { S_IN = 0, S_OUT, S_OUT2, S_TMP1, S_TMP2, S_TAPS, S_WORK, S_COUNT };
+
+int fail(ifm_ctx *ctx, int code, const std::string &msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                        \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess) return fail(ctx, IFM_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+void *scratch(ifm_ctx *ctx, int slot, size_t bytes) {
+  if (ctx->slots.size() < S_COUNT) ctx->slots.resize(S_COUNT, {nullptr, 0});
+  auto &s = ctx->slots[slot];
+  if (bytes == 0) bytes = 16;
+  if (s.second < bytes) {
+    if (s.first) {
+      (void)hipStreamSynchronize(ctx->stream);
+      (void)hipFree(s.first);
+      s.first = nullptr;
+      s.second = 0;
+    }
+    size_t want = bytes + bytes / 8;
+    if (hipMalloc(&s.first, want) != hipSuccess) {
+      s.first = nullptr;
+      return nullptr;
+    }
+    s.second = want;
+  }
+  return s.first;
+}
+
+// Canonical geometry: perm[m] = user axis stored at canonical position m.
+struct Geo {
+  int nd;          // user ndim
+  int perm[MAXD];  // -1 = padding axis
+  int inv[MAXD];   // user axis -> canonical position
+};
+
+bool valid_dtype(int dt) { return dt >= IFM_U8 && dt <= IFM_F16; }
+
+// Finds the axis permutation that makes `a` dense column-major.
+bool layout_of(const ifm_array &a, Geo &g) {
+  g.nd = a.ndim;
+  std::vector<int> big, unit;
+  for (int d = 0; d < a.ndim; ++d) (a.size[d] == 1 ? unit : big).push_back(d);
+  std::stable_sort(big.begin(), big.end(), [&](int x, int y) { return a.stride[x] < a.stride[y]; });
+  int m = 0;
+  for (int d : big) g.perm[m++] = d;
+  for (int d : unit) g.perm[m++] = d;
+  for (; m < MAXD; ++m) g.perm[m] = -1;
+  int64_t expect = 1;
+  for (int d : big) {
+    if (a.stride[d] != expect) return false;
+    expect *= a.size[d];
+  }
+  for (int d = 0; d < MAXD; ++d) g.inv[d] = -1;
+  for (int k = 0; k < MAXD; ++k)
+    if (g.perm[k] >= 0) g.inv[g.perm[k]] = k;
+  return true;
+}
+
+bool dense_under(const ifm_array &a, const Geo &g) {
+  int64_t expect = 1;
+  for (int m = 0; m < MAXD; ++m) {
+    int d = g.perm[m];
+    if (d < 0) continue;
+    if (a.size[d] != 1 && a.stride[d] != expect) return false;
+    expect *= a.size[d];
+  }
+  return true;
+}
+
+int check_array(ifm_ctx *ctx, const ifm_array *a, const char *name) {
+  if (!a) return fail(ctx, IFM_E_ARG, std::string(name) + " is NULL");
+  if (a->ndim < 1 || a->ndim > MAXD)
+    return fail(ctx, IFM_E_UNSUPPORTED, std::string(name) + ": ndim must be 1..4");
+  if (!valid_dtype(a->dtype)) return fail(ctx, IFM_E_ARG, std::string(name) + ": unknown eltype");
+  for (int d = 0; d < a->ndim; ++d) {
+    if (a->size[d] < 0) return fail(ctx, IFM_E_ARG, std::string(name) + ": negative size");
+    if (a->size[d] > (int64_t)INT32_MAX / 2) return fail(ctx, IFM_E_UNSUPPORTED, std::string(name) + ": axis too long");
+  }
+  return IFM_OK;
+}
+
+int64_t count_of(const ifm_array &a) {
+  int64_t c = 1;
+  for (int d = 0; d < a.ndim; ++d) c *= a.size[d];
+  return c;
+}
+
+// A device view of a user array in canonical order; host arrays are staged.
+struct View {
+  Dense4 d;
+  const ifm_array *user = nullptr;
+  bool staged = false;
+  size_t bytes = 0;
+};
+
+int make_view(ifm_ctx *ctx, const ifm_array *a, const Geo &g, int slot, bool upload, View &v) {
+  v.user = a;
+  v.d.dtype = a->dtype;
+  for (int m = 0; m < MAXD; ++m) v.d.n[m] = g.perm[m] >= 0 && g.perm[m] < a->ndim ? a->size[g.perm[m]] : 1;
+  v.bytes = (size_t)count_of(*a) * dtype_size(a->dtype);
+  if (a->device) {
+    v.d.data = a->data;
+    v.staged = false;
+    return IFM_OK;
+  }
+  void *p = scratch(ctx, slot, v.bytes);
+  if (!p) return fail(ctx, IFM_E_NOMEM, "device allocation failed (staging)");
+  if (upload && v.bytes) HIPCHK(hipMemcpyAsync(p, a->data, v.bytes, hipMemcpyHostToDevice, ctx->stream));
+  v.d.data = p;
+  v.staged = true;
+  return IFM_OK;
+}
+
+int finish_view(ifm_ctx *ctx, View &v) {
+  if (v.staged && v.bytes) {
+    HIPCHK(hipMemcpyAsync(v.user->data, v.d.data, v.bytes, hipMemcpyDeviceToHost, ctx->stream));
+  }
+  return IFM_OK;
+}
+
+int sync_if_host(ifm_ctx *ctx, bool any_host) {
+  HIPCHK(hipGetLastError());
+  if (any_host) HIPCHK(hipStreamSynchronize(ctx->stream));
+  return IFM_OK;
+}
+
+// Canonical per-axis values from user-axis arrays (NULL -> default).
+void to_canon(const Geo &g, const int64_t *user, int64_t dflt, int64_t *out) {
+  for (int m = 0; m < MAXD; ++m) {
+    int d = g.perm[m];
+    out[m] = (d >= 0 && user) ? user[d] : dflt;
+  }
+}
+
+int check_region(ifm_ctx *ctx, const Dense4 &in, const Dense4 &out, const int64_t *of) {
+  for (int m = 0; m < MAXD; ++m) {
+    if (of[m] < 0 || of[m] + out.n[m] > in.n[m])
+      return fail(ctx, IFM_E_DIM, "output indices lie outside the image axes");
+  }
+  return IFM_OK;
+}
+
+int common_setup(ifm_ctx *ctx, const ifm_array *in, const ifm_array *out, Geo &g) {
+  int rc;
+  if ((rc = check_array(ctx, in, "in"))) return rc;
+  if ((rc = check_array(ctx, out, "out"))) return rc;
+  if (in->ndim != out->ndim) return fail(ctx, IFM_E_DIM, "in and out must have the same number of dimensions");
+  if (!layout_of(*in, g)) return fail(ctx, IFM_E_UNSUPPORTED, "in is not a dense array (copy it first)");
+  if (!dense_under(*out, g)) return fail(ctx, IFM_E_UNSUPPORTED, "out must be dense with the same axis order as in");
+  return IFM_OK;
+}
+
+int check_border(ifm_ctx *ctx, const ifm_border *b) {
+  if (!b) return fail(ctx, IFM_E_ARG, "border is NULL");
+  if (b->style < IFM_REPLICATE || b->style > IFM_FILL) return fail(ctx, IFM_E_ARG, "border style unrecognized");
+  return IFM_OK;
+}
+
+// Reflect has period 2n-2; the reference fails (DivideError) for n == 1.
+int check_reflect(ifm_ctx *ctx, const ifm_border *b, const Dense4 &in, const bool *padded_axis) {
+  if (b->style != IFM_REFLECT) return IFM_OK;
+  for (int m = 0; m < MAXD; ++m)
+    if (padded_axis[m] && in.n[m] == 1)
+      return fail(ctx, IFM_E_ARG, "reflect padding of an axis of length 1 (zero period)");
+  return IFM_OK;
+}
+
+bool is_float_out(int dt) { return dt == IFM_F32 || dt == IFM_F64 || dt == IFM_BF16 || dt == IFM_F16; }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char *ifm_version(void) { return "ifm 0.1 gfx950"; }
+
+int ifm_device_count(int *count) {
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+  if (count) *count = c;
+  return IFM_OK;
+}
+
+int ifm_ctx_create(int device, void *stream, ifm_ctx **out) {
+  if (!out) return IFM_E_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return IFM_E_DEVICE;
+  if (device < 0 || device >= n) return IFM_E_ARG;
+  if (hipSetDevice(device) != hipSuccess) return IFM_E_DEVICE;
+  ifm_ctx *ctx = new ifm_ctx();
+  ctx->device = device;
+  ctx->stream = (hipStream_t)stream;
+  ctx->slots.resize(S_COUNT, {nullptr, 0});
+  *out = ctx;
+  return IFM_OK;
+}
+
+int ifm_ctx_destroy(ifm_ctx *ctx) {
+  if (!ctx) return IFM_OK;
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto &s : ctx->slots)
+    if (s.first) (void)hipFree(s.first);
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  for (auto &e : ctx->taps) (void)hipFree(e.dev);
+  delete ctx;
+  return IFM_OK;
+}
+
+int ifm_ctx_set_stream(ifm_ctx *ctx, void *stream) {
+  if (!ctx) return IFM_E_ARG;
+  ctx->stream = (hipStream_t)stream;
+  return IFM_OK;
+}
+
+int ifm_ctx_synchronize(ifm_ctx *ctx) {
+  if (!ctx) return IFM_E_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return IFM_OK;
+}
+
+const char *ifm_last_error(const ifm_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int ifm_device_alloc(ifm_ctx *ctx, uint64_t bytes, void **ptr) {
+  if (!ctx || !ptr) return IFM_E_ARG;
+  *ptr = nullptr;
+  if (hipMalloc(ptr, bytes ? bytes : 16) != hipSuccess) return fail(ctx, IFM_E_NOMEM, "hipMalloc failed");
+  return IFM_OK;
+}
+
+int ifm_device_free(ifm_ctx *ctx, void *ptr) {
+  if (!ctx) return IFM_E_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (ptr) HIPCHK(hipFree(ptr));
+  return IFM_OK;
+}
+
+int ifm_memcpy_h2d(ifm_ctx *ctx, void *dst, const void *src, uint64_t bytes) {
+  if (!ctx) return IFM_E_ARG;
+  if (bytes) HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return IFM_OK;
+}
+
+int ifm_memcpy_d2h(ifm_ctx *ctx, void *dst, const void *src, uint64_t bytes) {
+  if (!ctx) return IFM_E_ARG;
+  if (bytes) HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return IFM_OK;
+}
+
+int ifm_memset_d(ifm_ctx *ctx, void *dst, int value, uint64_t bytes) {
+  if (!ctx) return IFM_E_ARG;
+  if (bytes) HIPCHK(hipMemsetAsync(dst, value, bytes, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return IFM_OK;
+}
+
+int ifm_timer_start(ifm_ctx *ctx) {
+  if (!ctx) return IFM_E_ARG;
+  if (!ctx->ev0) HIPCHK(hipEventCreate(&ctx->ev0));
+  if (!ctx->ev1) HIPCHK(hipEventCreate(&ctx->ev1));
+  HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+  return IFM_OK;
+}
+
+int ifm_timer_stop(ifm_ctx *ctx, float *ms) {
+  if (!ctx || !ctx->ev0) return IFM_E_ARG;
+  HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+  HIPCHK(hipEventSynchronize(ctx->ev1));
+  float t = 0.f;
+  HIPCHK(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+  if (ms) *ms = t;
+  return IFM_OK;
+}
+
+// ---- padarray --------------------------------------------------------------
+int ifm_padarray(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const ifm_border *b) {
+  if (!ctx) return IFM_E_ARG;
+  Geo g;
+  int rc;
+  if ((rc = common_setup(ctx, in, out, g))) return rc;
+  if ((rc = check_border(ctx, b))) return rc;
+  for (int d = 0; d < in->ndim; ++d) {
+    if (b->lo[d] < 0 || b->hi[d] < 0) return fail(ctx, IFM_E_ARG, "negative padding");
+    if (out->size[d] != b->lo[d] + in->size[d] + b->hi[d])
+      return fail(ctx, IFM_E_DIM, "axes(dst) == axes(src) must hold");
+    if (b->style == IFM_REFLECT && in->size[d] == 1 && (b->lo[d] || b->hi[d]))
+      return fail(ctx, IFM_E_ARG, "reflect padding of an axis of length 1 (zero period)");
+    if (b->style != IFM_FILL && in->size[d] == 0 && (b->lo[d] || b->hi[d]))
+      return fail(ctx, IFM_E_ARG, "cannot pad an empty axis");
+  }
+  if (in->dtype != out->dtype && !is_float_out(out->dtype))
+    return fail(ctx, IFM_E_UNSUPPORTED, "padarray: conversion to an integer eltype");
+  View vi, vo;
+  if ((rc = make_view(ctx, in, g, S_IN, true, vi))) return rc;
+  if ((rc = make_view(ctx, out, g, S_OUT, false, vo))) return rc;
+  int64_t lo[MAXD];
+  to_canon(g, b->lo, 0, lo);
+  HIPCHK(launch_pad(vi.d, vo.d, lo, b->style, b->fill, ctx->stream));
+  if ((rc = finish_view(ctx, vo))) return rc;
+  return sync_if_host(ctx, !in->device || !out->device);
+}
+
+// ---- FIR, separable ------------------------------------------------------------
+int ifm_imfilter_fir(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const int64_t *out_first,
+                     const ifm_factor *factors, int nfactors, const ifm_border *b) {
+  if (!ctx) return IFM_E_ARG;
+  Geo g;
+  int rc;
+  if ((rc = common_setup(ctx, in, out, g))) return rc;
+  if ((rc = check_border(ctx, b))) return rc;
+  if (!is_float_out(out->dtype)) return fail(ctx, IFM_E_UNSUPPORTED, "imfilter: output eltype must be floating point");
+  if (nfactors < 0 || (nfactors > 0 && !factors)) return fail(ctx, IFM_E_ARG, "bad factor list");
+  // Merge the factors per axis: consecutive correlations compose to the
+  // correlation with their convolution (first offsets add).
+  std::vector<double> taps[MAXD];
+  int64_t first[MAXD] = {0, 0, 0, 0};
+  bool has[MAXD] = {false, false, false, false};
+  for (int f = 0; f < nfactors; ++f) {
+    const ifm_factor &F = factors[f];
+    if (F.axis < 0 || F.axis >= in->ndim) return fail(ctx, IFM_E_DIM, "factor axis beyond the image dimensions");
+    if (F.len < 1 || !F.taps) return fail(ctx, IFM_E_ARG, "empty factor");
+    if (F.len == 1 && F.first == 0 && F.taps[0] == 1.0) continue;  // iscopy
+    int m = g.inv[F.axis];
+    if (!has[m]) {
+      taps[m].assign(F.taps, F.taps + F.len);
+      first[m] = F.first;
+      has[m] = true;
+    } else {
+      std::vector<double> c(taps[m].size() + F.len - 1, 0.0);
+      for (size_t i = 0; i < taps[m].size(); ++i)
+        for (int j = 0; j < F.len; ++j) c[i + j] += taps[m][i] * F.taps[j];
+      taps[m].swap(c);
+      first[m] += F.first;
+    }
+  }
+  for (int m = 0; m < MAXD; ++m)
+    if (has[m] && taps[m].size() > (size_t)MAXTAPS)
+      return fail(ctx, IFM_E_UNSUPPORTED, "factor longer than 128 taps: use Algorithm.FFT()");
+  View vi, vo;
+  if ((rc = make_view(ctx, in, g, S_IN, true, vi))) return rc;
+  if ((rc = make_view(ctx, out, g, S_OUT, false, vo))) return rc;
+  int64_t of[MAXD];
+  to_canon(g, out_first, 0, of);
+  if ((rc = check_region(ctx, vi.d, vo.d, of))) return rc;
+  if ((rc = check_reflect(ctx, b, vi.d, has))) return rc;
+  for (int m = 0; m < MAXD; ++m)
+    if (has[m] && vi.d.n[m] == 0 && b->style != IFM_FILL) return fail(ctx, IFM_E_ARG, "cannot pad an empty axis");
+
+  std::vector<int> axes;
+  for (int m = 0; m < MAXD; ++m)
+    if (has[m]) axes.push_back(m);
+  auto pass_of = [&](int m) {
+    FirPass p;
+    p.axis = m;
+    p.len = (int)taps[m].size();
+    p.first = (int)first[m];
+    for (int t = 0; t < p.len; ++t) p.taps[t] = (float)taps[m][t];
+    return p;
+  };
+  auto tapsum = [&](int m) {
+    double s = 0;
+    for (double t : taps[m]) s += t;
+    return s;
+  };
+  const int style = b->style;
+  double fill = b->fill;
+  hipStream_t st = ctx->stream;
+  if (vo.d.count() > 0) {
+    if (axes.empty()) {
+      FirPass p;
+      p.axis = 0;
+      p.len = 1;
+      p.first = 0;
+      p.taps[0] = 1.f;
+      int64_t zero[MAXD] = {0, 0, 0, 0};
+      HIPCHK(launch_fir1d(vi.d, zero, vo.d, of, vi.d.n, p, style, (float)fill, st));
+    } else {
+      // Groups: a fused (0,1) pair when both are present and short enough,
+      // then single passes for the remaining axes.
+      std::vector<std::vector<int>> groups;
+      size_t i = 0;
+      if (axes.size() >= 2 && axes[0] == 0 && axes[1] == 1 && taps[0].size() <= 64 && taps[1].size() <= 64) {
+        groups.push_back({0, 1});
+        i = 2;
+      }
+      for (; i < axes.size(); ++i) groups.push_back({axes[i]});
+      // Source of the current group: covers out region on filtered axes and the
+      // whole image on the others.
+      Dense4 src = vi.d;
+      int64_t src_first[MAXD] = {0, 0, 0, 0};
+      bool done[MAXD] = {false, false, false, false};
+      for (size_t gi = 0; gi < groups.size(); ++gi) {
+        const bool last = gi + 1 == groups.size();
+        for (int m : groups[gi]) done[m] = true;
+        Dense4 dst;
+        int64_t dst_first[MAXD];
+        if (last) {
+          dst = vo.d;
+          for (int m = 0; m < MAXD; ++m) dst_first[m] = of[m];
+        } else {
+          dst.dtype = IFM_F32;
+          for (int m = 0; m < MAXD; ++m) {
+            bool later = false;
+            for (size_t gj = gi + 1; gj < groups.size(); ++gj)
+              for (int x : groups[gj]) later |= x == m;
+            dst.n[m] = later ? vi.d.n[m] : vo.d.n[m];
+            dst_first[m] = later ? 0 : of[m];
+          }
+          dst.data = scratch(ctx, (gi & 1) ? S_TMP2 : S_TMP1, (size_t)dst.count() * sizeof(float));
+          if (!dst.data) return fail(ctx, IFM_E_NOMEM, "device allocation failed (temporary)");
+        }
+        if (groups[gi].size() == 2) {
+          // sep2d reads planes p2+f2, p3+f3 of its source; the source is the
+          // input here (first group), so f2/f3 are the destination offsets.
+          HIPCHK(launch_sep2d(src, dst, dst_first, pass_of(0), pass_of(1), style, (float)fill, st));
+          fill *= tapsum(0) * tapsum(1);
+        } else {
+          int m = groups[gi][0];
+          HIPCHK(launch_fir1d(src, src_first, dst, dst_first, vi.d.n, pass_of(m), style, (float)fill, st));
+          fill *= tapsum(m);
+        }
+        src = dst;
+        for (int m = 0; m < MAXD; ++m) src_first[m] = dst_first[m];
+      }
+    }
+  }
+  if ((rc = finish_view(ctx, vo))) return rc;
+  return sync_if_host(ctx, !in->device || !out->device);
+}
+
+// ---- FIR, dense --------------------------------------------------------------------
+static int device_taps(ifm_ctx *ctx, const std::vector<float> &t, const float **out) {
+  uint64_t h = 1469598103934665603ull;  // FNV-1a over the tap bytes
+  const unsigned char *p = (const unsigned char *)t.data();
+  for (size_t i = 0; i < t.size() * sizeof(float); ++i) h = (h ^ p[i]) * 1099511628211ull;
+  for (auto &e : ctx->taps)
+    if (e.hash == h && e.n == t.size()) {
+      *out = (const float *)e.dev;
+      return IFM_OK;
+    }
+  if (ctx->taps.size() >= 64) {  // bounded cache: drop the oldest entry
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipFree(ctx->taps.front().dev));
+    ctx->taps.erase(ctx->taps.begin());
+  }
+  void *d = nullptr;
+  if (hipMalloc(&d, t.size() * sizeof(float)) != hipSuccess) return fail(ctx, IFM_E_NOMEM, "device allocation failed (taps)");
+  HIPCHK(hipMemcpy(d, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
+  ctx->taps.push_back({h, t.size(), d});
+  *out = (const float *)d;
+  return IFM_OK;
+}
+
+static int dense_kernel_canon(ifm_ctx *ctx, const ifm_kernel *k, const Geo &g, int img_ndim, DenseKernel &dk) {
+  if (!k || !k->taps) return fail(ctx, IFM_E_ARG, "kernel is NULL");
+  if (k->ndim < 0 || k->ndim > img_ndim) return fail(ctx, IFM_E_DIM, "kernel has more dimensions than the image");
+  int64_t ks[MAXD], kf[MAXD];
+  for (int d = 0; d < MAXD; ++d) {
+    ks[d] = d < k->ndim ? k->size[d] : 1;
+    kf[d] = d < k->ndim ? k->first[d] : 0;
+    if (ks[d] < 1) return fail(ctx, IFM_E_ARG, "empty kernel");
+    if (ks[d] > 1024) return fail(ctx, IFM_E_UNSUPPORTED, "kernel axis longer than 1024");
+  }
+  dk.ndim = MAXD;
+  for (int m = 0; m < MAXD; ++m) {
+    int d = g.perm[m];
+    dk.n[m] = d >= 0 ? (int)ks[d] : 1;
+    dk.first[m] = d >= 0 ? (int)kf[d] : 0;
+  }
+  int64_t total = (int64_t)dk.n[0] * dk.n[1] * dk.n[2] * dk.n[3];
+  dk.taps.resize(total);
+  // user strides of the column-major kernel
+  int64_t us[MAXD], acc = 1;
+  for (int d = 0; d < MAXD; ++d) {
+    us[d] = acc;
+    acc *= ks[d];
+  }
+  int64_t q = 0;
+  for (int i3 = 0; i3 < dk.n[3]; ++i3)
+    for (int i2 = 0; i2 < dk.n[2]; ++i2)
+      for (int i1 = 0; i1 < dk.n[1]; ++i1)
+        for (int i0 = 0; i0 < dk.n[0]; ++i0, ++q) {
+          int idx[MAXD] = {i0, i1, i2, i3};
+          int64_t src = 0;
+          for (int m = 0; m < MAXD; ++m)
+            if (g.perm[m] >= 0) src += idx[m] * us[g.perm[m]];
+          dk.taps[q] = (float)k->taps[src];
+        }
+  return IFM_OK;
+}
+
+int ifm_imfilter_dense(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const int64_t *out_first,
+                       const ifm_kernel *kernel, const ifm_border *b, int precision) {
+  if (!ctx) return IFM_E_ARG;
+  Geo g;
+  int rc;
+  if ((rc = common_setup(ctx, in, out, g))) return rc;
+  if ((rc = check_border(ctx, b))) return rc;
+  if (!is_float_out(out->dtype)) return fail(ctx, IFM_E_UNSUPPORTED, "imfilter: output eltype must be floating point");
+  DenseKernel dk;
+  if ((rc = dense_kernel_canon(ctx, kernel, g, in->ndim, dk))) return rc;
+  View vi, vo;
+  if ((rc = make_view(ctx, in, g, S_IN, true, vi))) return rc;
+  if ((rc = make_view(ctx, out, g, S_OUT, false, vo))) return rc;
+  int64_t of[MAXD];
+  to_canon(g, out_first, 0, of);
+  if ((rc = check_region(ctx, vi.d, vo.d, of))) return rc;
+  bool padded[MAXD];
+  for (int m = 0; m < MAXD; ++m) padded[m] = dk.n[m] > 1 || dk.first[m] != 0;
+  if ((rc = check_reflect(ctx, b, vi.d, padded))) return rc;
+  bool twod = dk.n[2] == 1 && dk.n[3] == 1 && dk.first[2] == 0 && dk.first[3] == 0;
+  bool mfma = false;
+  if (precision == IFM_PREC_BF16X3) mfma = twod && dense_mfma_supported(dk, vi.d);
+  else if (precision == IFM_PREC_AUTO)
+    mfma = twod && dk.n[0] >= 15 && dk.n[1] >= 15 && dense_mfma_supported(dk, vi.d);
+  else if (precision != IFM_PREC_FP32) return fail(ctx, IFM_E_ARG, "unknown precision");
+  const float *dtaps = nullptr;
+  if ((rc = device_taps(ctx, dk.taps, &dtaps))) return rc;
+  if (mfma)
+    HIPCHK(launch_dense_mfma(vi.d, vo.d, of, dk, (void *)dtaps, dk.taps.size() * sizeof(float), b->style, (float)b->fill,
+                             ctx->stream));
+  else
+    HIPCHK(launch_dense_valu(vi.d, vo.d, of, dk, dtaps, b->style, (float)b->fill, ctx->stream));
+  if ((rc = finish_view(ctx, vo))) return rc;
+  return sync_if_host(ctx, !in->device || !out->device);
+}
+
+// ---- FFT ---------------------------------------------------------------------------
+static int64_t nextprod23(int64_t n) {
+  int64_t best = INT64_MAX;
+  for (int64_t p2 = 1; p2 < 2 * n + 2; p2 *= 2)
+    for (int64_t p = p2; p < 2 * n + 2; p *= 3)
+      if (p >= n && p < best) best = p;
+  return best;
+}
+
+int ifm_imfilter_fft(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const int64_t *out_first,
+                     const ifm_kernel *kernel, const ifm_border *b) {
+  if (!ctx) return IFM_E_ARG;
+  Geo g;
+  int rc;
+  if ((rc = common_setup(ctx, in, out, g))) return rc;
+  if ((rc = check_border(ctx, b))) return rc;
+  if (!is_float_out(out->dtype)) return fail(ctx, IFM_E_UNSUPPORTED, "imfilter: output eltype must be floating point");
+  DenseKernel dk;
+  if ((rc = dense_kernel_canon(ctx, kernel, g, in->ndim, dk))) return rc;
+  View vi, vo;
+  if ((rc = make_view(ctx, in, g, S_IN, true, vi))) return rc;
+  if ((rc = make_view(ctx, out, g, S_OUT, false, vo))) return rc;
+  int64_t of[MAXD];
+  to_canon(g, out_first, 0, of);
+  if ((rc = check_region(ctx, vi.d, vo.d, of))) return rc;
+  // Axes with a kernel extent are padded to nextprod([2,3], n + K) and
+  // transformed; a kernel with a single tap at 0 on an axis is the identity.
+  int64_t N[MAXD], lo[MAXD];
+  bool padded[MAXD];
+  for (int m = 0; m < MAXD; ++m) {
+    padded[m] = dk.n[m] > 1 || dk.first[m] != 0;
+    if (padded[m]) {
+      int64_t last = dk.first[m] + dk.n[m] - 1;
+      lo[m] = dk.first[m] < 0 ? -dk.first[m] : 0;
+      (void)last;
+      N[m] = nextprod23(vi.d.n[m] + dk.n[m]);
+      if (N[m] > 8192) return fail(ctx, IFM_E_UNSUPPORTED, "FFT length above 8192 on one axis");
+    } else {
+      lo[m] = 0;
+      N[m] = vi.d.n[m];
+    }
+  }
+  if (N[3] != 1) return fail(ctx, IFM_E_UNSUPPORTED, "FFT filtering supports up to 3 dimensions");
+  if ((rc = check_reflect(ctx, b, vi.d, padded))) return rc;
+  // The FFT kernel transforms every one of the first nd axes; unpadded axes
+  // between padded ones are transformed too (a length-n DFT round trip).
+  int nd = 1;
+  for (int m = 0; m < 3; ++m)
+    if (padded[m]) nd = m + 1;
+  for (int m = 0; m < nd; ++m)
+    if (N[m] > 8192) return fail(ctx, IFM_E_UNSUPPORTED, "FFT length above 8192 on one axis");
+  for (int m = 0; m < nd; ++m) {
+    int64_t x = N[m];
+    while (x % 2 == 0) x /= 2;
+    while (x % 3 == 0) x /= 3;
+    if (x != 1) {  // an unpadded axis of awkward length inside the transformed block
+      N[m] = nextprod23(vi.d.n[m] + dk.n[m]);
+      padded[m] = true;
+      lo[m] = dk.first[m] < 0 ? -dk.first[m] : 0;
+      if (N[m] > 8192) return fail(ctx, IFM_E_UNSUPPORTED, "FFT length above 8192 on one axis");
+    }
+  }
+  size_t ws = fft_workspace(N, 3) + dk.taps.size() * sizeof(float) + 256;
+  void *work = scratch(ctx, S_WORK, ws);
+  if (!work) return fail(ctx, IFM_E_NOMEM, "device allocation failed (FFT workspace)");
+  if (vo.d.count() > 0)
+    HIPCHK(launch_fft_filter(vi.d, vo.d, of, dk, N, nd, lo, b->style, (float)b->fill, work, ctx->stream));
+  if ((rc = finish_view(ctx, vo))) return rc;
+  return sync_if_host(ctx, !in->device || !out->device);
+}
+
+// ---- IIR ------------------------------------------------------------------------------
+int ifm_imfilter_iir(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const ifm_iir *factors, int nfactors,
+                     const ifm_border *b) {
+  if (!ctx) return IFM_E_ARG;
+  Geo g;
+  int rc;
+  if ((rc = common_setup(ctx, in, out, g))) return rc;
+  if ((rc = check_border(ctx, b))) return rc;
+  if (!is_float_out(out->dtype)) return fail(ctx, IFM_E_UNSUPPORTED, "imfilter: output eltype must be floating point");
+  for (int d = 0; d < in->ndim; ++d)
+    if (in->size[d] != out->size[d]) return fail(ctx, IFM_E_DIM, "IIR output must have the image's axes");
+  if (nfactors < 0 || (nfactors > 0 && !factors)) return fail(ctx, IFM_E_ARG, "bad factor list");
+  for (int f = 0; f < nfactors; ++f) {
+    if (factors[f].axis < 0 || factors[f].axis >= in->ndim)
+      return fail(ctx, IFM_E_DIM, "cannot have more kernels than dimensions");
+    if (in->size[factors[f].axis] <= 3)
+      return fail(ctx, IFM_E_DIM, "size of img along the filtered dimension is too small for the IIR kernel of length 3");
+  }
+  View vi, vo;
+  if ((rc = make_view(ctx, in, g, S_IN, true, vi))) return rc;
+  if ((rc = make_view(ctx, out, g, S_OUT, false, vo))) return rc;
+  hipStream_t st = ctx->stream;
+  int isfill = b->style == IFM_FILL;
+  std::vector<int> live;
+  for (int f = 0; f < nfactors; ++f) {
+    const ifm_iir &k = factors[f];
+    bool copy = k.a[0] == 0 && k.a[1] == 0 && k.a[2] == 0 && k.b[0] == 0 && k.b[1] == 0 && k.b[2] == 0 &&
+                k.scale == 1.0;  // iscopy(::TriggsSdika)
+    if (!copy) live.push_back(f);
+  }
+  const int64_t cnt = vi.d.count();
+  if (cnt > 0) {
+    if (live.empty()) {
+      FirPass p;
+      p.axis = 0;
+      p.len = 1;
+      p.first = 0;
+      p.taps[0] = 1.f;
+      int64_t zero[MAXD] = {0, 0, 0, 0};
+      if (vi.d.data != vo.d.data || vi.d.dtype != vo.d.dtype)
+        HIPCHK(launch_fir1d(vi.d, zero, vo.d, zero, vi.d.n, p, b->style, (float)b->fill, st));
+    } else {
+      Dense4 src = vi.d;
+      for (size_t s = 0; s < live.size(); ++s) {
+        const ifm_iir &k = factors[live[s]];
+        const bool last = s + 1 == live.size();
+        Dense4 dst = vi.d;
+        dst.dtype = IFM_F64;
+        if (last && vo.d.dtype == IFM_F64) {
+          dst.data = vo.d.data;
+        } else {
+          dst.data = scratch(ctx, (s & 1) ? S_TMP2 : S_TMP1, (size_t)cnt * sizeof(double));
+          if (!dst.data) return fail(ctx, IFM_E_NOMEM, "device allocation failed (temporary)");
+        }
+        IirCoef c;
+        for (int i = 0; i < 3; ++i) {
+          c.a[i] = k.a[i];
+          c.b[i] = k.b[i];
+        }
+        c.scale = k.scale;
+        for (int i = 0; i < 9; ++i) c.M[i] = k.M[i];
+        HIPCHK(launch_iir(src, dst, g.inv[k.axis], c, isfill, b->fill, st));
+        src = dst;
+      }
+      if (src.data != vo.d.data) {
+        FirPass p;
+        p.axis = 0;
+        p.len = 1;
+        p.first = 0;
+        p.taps[0] = 1.f;
+        int64_t zero[MAXD] = {0, 0, 0, 0};
+        HIPCHK(launch_fir1d(src, zero, vo.d, zero, vi.d.n, p, IFM_REPLICATE, 0.f, st));
+      }
+    }
+  }
+  if ((rc = finish_view(ctx, vo))) return rc;
+  return sync_if_host(ctx, !in->device || !out->device);
+}
+
+// ---- mapwindow ---------------------------------------------------------------------------
+static int window_setup(ifm_ctx *ctx, const Geo &g, const int64_t *wf, const int64_t *wl, int *cwf, int *cwl) {
+  if (!wf || !wl) return fail(ctx, IFM_E_ARG, "window is NULL");
+  for (int m = 0; m < MAXD; ++m) {
+    int d = g.perm[m];
+    cwf[m] = d >= 0 ? (int)wf[d] : 0;
+    cwl[m] = d >= 0 ? (int)wl[d] : 1;
+    if (cwl[m] < 1) return fail(ctx, IFM_E_ARG, "empty window");
+    if (cwl[m] > 4096) return fail(ctx, IFM_E_UNSUPPORTED, "window longer than 4096");
+  }
+  return IFM_OK;
+}
+
+static int window_reflect_check(ifm_ctx *ctx, int style, const Dense4 &in, const int *wf, const int *wl) {
+  if (style != IFM_REFLECT) return IFM_OK;
+  for (int m = 0; m < MAXD; ++m)
+    if (in.n[m] == 1 && (wf[m] != 0 || wl[m] != 1))
+      return fail(ctx, IFM_E_ARG, "reflect padding of an axis of length 1 (zero period)");
+  return IFM_OK;
+}
+
+int ifm_mapwindow_median(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const int64_t *out_first,
+                         const int64_t *win_first, const int64_t *win_len, const ifm_border *b) {
+  if (!ctx) return IFM_E_ARG;
+  Geo g;
+  int rc;
+  if ((rc = common_setup(ctx, in, out, g))) return rc;
+  if ((rc = check_border(ctx, b))) return rc;
+  if (in->dtype != out->dtype) return fail(ctx, IFM_E_ARG, "median output holds the input eltype");
+  int wf[MAXD], wl[MAXD];
+  if ((rc = window_setup(ctx, g, win_first, win_len, wf, wl))) return rc;
+  int64_t L = (int64_t)wl[0] * wl[1] * wl[2] * wl[3];
+  if (L > 4096) return fail(ctx, IFM_E_UNSUPPORTED, "median window larger than 4096 elements");
+  View vi, vo;
+  if ((rc = make_view(ctx, in, g, S_IN, true, vi))) return rc;
+  if ((rc = make_view(ctx, out, g, S_OUT, false, vo))) return rc;
+  int64_t of[MAXD];
+  to_canon(g, out_first, 0, of);
+  if ((rc = check_region(ctx, vi.d, vo.d, of))) return rc;
+  if ((rc = window_reflect_check(ctx, b->style, vi.d, wf, wl))) return rc;
+  if (vi.d.count() == 0 && vo.d.count() > 0) return fail(ctx, IFM_E_ARG, "empty image");
+  HIPCHK(launch_median(vi.d, vo.d, of, wf, wl, b->style, b->fill, ctx->stream));
+  if ((rc = finish_view(ctx, vo))) return rc;
+  return sync_if_host(ctx, !in->device || !out->device);
+}
+
+static int minmax_common(ifm_ctx *ctx, const ifm_array *in, ifm_array *omin, ifm_array *omax, const int64_t *out_first,
+                         const int *wf_user_canon_unused, const int64_t *win_first, const int64_t *win_len, int style,
+                         double fill, bool lemire) {
+  (void)wf_user_canon_unused;
+  Geo g;
+  int rc;
+  ifm_array *ref = omin ? omin : omax;
+  if (!ref) return fail(ctx, IFM_E_ARG, "no output requested");
+  if ((rc = common_setup(ctx, in, ref, g))) return rc;
+  if (omin && omax) {
+    if ((rc = check_array(ctx, omax, "omax"))) return rc;
+    for (int d = 0; d < in->ndim; ++d)
+      if (omin->size[d] != omax->size[d]) return fail(ctx, IFM_E_DIM, "omin and omax differ in size");
+    if (!dense_under(*omax, g)) return fail(ctx, IFM_E_UNSUPPORTED, "omax must share the layout of in");
+  }
+  if ((omin && omin->dtype != in->dtype) || (omax && omax->dtype != in->dtype))
+    return fail(ctx, IFM_E_ARG, "min/max outputs hold the input eltype");
+  int wf[MAXD], wl[MAXD];
+  if ((rc = window_setup(ctx, g, win_first, win_len, wf, wl))) return rc;
+  View vi, vmin, vmax;
+  if ((rc = make_view(ctx, in, g, S_IN, true, vi))) return rc;
+  if (omin && (rc = make_view(ctx, omin, g, S_OUT, false, vmin))) return rc;
+  if (omax && (rc = make_view(ctx, omax, g, S_OUT2, false, vmax))) return rc;
+  int64_t of[MAXD];
+  to_canon(g, out_first, 0, of);
+  Dense4 &r = omin ? vmin.d : vmax.d;
+  if ((rc = check_region(ctx, vi.d, r, of))) return rc;
+  if (!lemire && (rc = window_reflect_check(ctx, style, vi.d, wf, wl))) return rc;
+  if (vi.d.count() == 0 && r.count() > 0) return fail(ctx, IFM_E_ARG, "empty image");
+  size_t ws = minmax_workspace(vi.d, r.n);
+  void *work = scratch(ctx, S_WORK, ws);
+  if (!work) return fail(ctx, IFM_E_NOMEM, "device allocation failed (min/max workspace)");
+  HIPCHK(launch_minmax(vi.d, omin ? &vmin.d : nullptr, omax ? &vmax.d : nullptr, of, wf, wl,
+                       style == IFM_FILL ? 1 : 0, fill, work, ctx->stream));
+  if (omin && (rc = finish_view(ctx, vmin))) return rc;
+  if (omax && (rc = finish_view(ctx, vmax))) return rc;
+  bool host = !in->device || (omin && !omin->device) || (omax && !omax->device);
+  return sync_if_host(ctx, host);
+}
+
+int ifm_mapwindow_minmax(ifm_ctx *ctx, const ifm_array *in, ifm_array *omin, ifm_array *omax, const int64_t *out_first,
+                         const int64_t *win_first, const int64_t *win_len, const ifm_border *b) {
+  if (!ctx) return IFM_E_ARG;
+  int rc;
+  if ((rc = check_border(ctx, b))) return rc;
+  return minmax_common(ctx, in, omin, omax, out_first, nullptr, win_first, win_len, b->style, b->fill, false);
+}
+
+int ifm_extrema_filter(ifm_ctx *ctx, const ifm_array *in, ifm_array *omin, ifm_array *omax, const int64_t *window) {
+  if (!ctx) return IFM_E_ARG;
+  if (!window || !in) return fail(ctx, IFM_E_ARG, "window is NULL");
+  int64_t wf[MAXD] = {0, 0, 0, 0}, wl[MAXD] = {1, 1, 1, 1};
+  for (int d = 0; d < in->ndim && d < MAXD; ++d) {
+    if (window[d] < 1) return fail(ctx, IFM_E_ARG, "window entries must be positive");
+    // Lemire with the delayed cache covers offsets -(w>>1) .. w-1-(w>>1)
+    wf[d] = -(window[d] >> 1);
+    wl[d] = window[d];
+  }
+  return minmax_common(ctx, in, omin, omax, nullptr, nullptr, wf, wl, IFM_REPLICATE, 0.0, true);
+}
+
+}  // extern "C"

@@ -1,0 +1,220 @@
+// ifm_internal.h — shared host/device definitions of libifm (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+#include <stdint.h>
+#include <string.h>
+#include <string>
+#include <vector>
+
+#include "../../include/ifm.h"
+
+namespace ifm {
+
+constexpr int MAXD = IFM_MAXDIM;
+constexpr int MAXTAPS = 128;  // per-axis tap limit of the FIR kernels (longer -> FFT)
+
+// ---------------------------------------------------------------------------
+// Border index map (src/border.jl:564-590, periodic form): the source index in
+// [0, n) for padded coordinate i, or -1 for Fill.  Reflect needs n >= 2; the
+// host rejects n == 1 as the reference does (zero period).
+// ---------------------------------------------------------------------------
+__host__ __device__ inline int imod(int x, int m) {
+  int r = x % m;
+  return r < 0 ? r + m : r;
+}
+
+__host__ __device__ inline int bmap(int style, int i, int n) {
+  if ((unsigned)i < (unsigned)n) return i;
+  switch (style) {
+  case IFM_REPLICATE: return i < 0 ? 0 : n - 1;
+  case IFM_CIRCULAR: return imod(i, n);
+  case IFM_SYMMETRIC: { int j = imod(i, 2 * n); return j < n ? j : 2 * n - 1 - j; }
+  case IFM_REFLECT: {
+    if (n == 1) return 0;  // rejected on the host; keeps the index in bounds
+    int j = imod(i, 2 * n - 2);
+    return j < n ? j : 2 * n - 2 - j;
+  }
+  default: return -1;
+  }
+}
+
+// Element access with a runtime eltype: the branch is uniform over the grid.
+__device__ __forceinline__ float bf16_to_f32(uint16_t b) { return __uint_as_float(((uint32_t)b) << 16); }
+__device__ __forceinline__ uint16_t f32_to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);  // quiet NaN
+  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+  return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ float load_f(const void *p, int64_t i, int dt) {
+  switch (dt) {
+  case IFM_U8: return (float)((const uint8_t *)p)[i];
+  case IFM_N0F8: return (float)((const uint8_t *)p)[i] * (1.0f / 255.0f);
+  case IFM_F32: return ((const float *)p)[i];
+  case IFM_F64: return (float)((const double *)p)[i];
+  case IFM_BF16: return bf16_to_f32(((const uint16_t *)p)[i]);
+  default: return __half2float(((const __half *)p)[i]);
+  }
+}
+
+__device__ __forceinline__ double load_d(const void *p, int64_t i, int dt) {
+  switch (dt) {
+  case IFM_U8: return (double)((const uint8_t *)p)[i];
+  case IFM_N0F8: return (double)((const uint8_t *)p)[i] / 255.0;
+  case IFM_F32: return (double)((const float *)p)[i];
+  case IFM_F64: return ((const double *)p)[i];
+  case IFM_BF16: return (double)bf16_to_f32(((const uint16_t *)p)[i]);
+  default: return (double)__half2float(((const __half *)p)[i]);
+  }
+}
+
+__device__ __forceinline__ void store_f(void *p, int64_t i, int dt, float v) {
+  switch (dt) {
+  case IFM_F32: ((float *)p)[i] = v; break;
+  case IFM_F64: ((double *)p)[i] = (double)v; break;
+  case IFM_BF16: ((uint16_t *)p)[i] = f32_to_bf16(v); break;
+  case IFM_F16: ((__half *)p)[i] = __float2half_rn(v); break;
+  default: break;  // integer outputs are produced by the order filters only
+  }
+}
+
+__device__ __forceinline__ void store_d(void *p, int64_t i, int dt, double v) {
+  switch (dt) {
+  case IFM_F32: ((float *)p)[i] = (float)v; break;
+  case IFM_F64: ((double *)p)[i] = v; break;
+  case IFM_BF16: ((uint16_t *)p)[i] = f32_to_bf16((float)v); break;
+  case IFM_F16: ((__half *)p)[i] = __float2half_rn((float)v); break;
+  default: break;
+  }
+}
+
+// Order keys: an unsigned integer whose order is Julia's isless on the
+// element type (NaN above +Inf, -0.0 below +0.0).  Codes of UInt8 / N0f8 are
+// their own keys.  key64 is used for Float64 only.
+__device__ __forceinline__ uint32_t key32(const void *p, int64_t i, int dt) {
+  switch (dt) {
+  case IFM_U8:
+  case IFM_N0F8: return ((const uint8_t *)p)[i];
+  case IFM_F32: {
+    uint32_t u = __float_as_uint(((const float *)p)[i]);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return 0xffffffffu;
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+  }
+  default: {  // BF16 / F16: 16-bit float layout
+    uint32_t u = ((const uint16_t *)p)[i];
+    uint32_t expmask = (dt == IFM_BF16) ? 0x7f80u : 0x7c00u;
+    if ((u & 0x7fffu) > expmask) return 0xffffu;
+    return (u & 0x8000u) ? (~u & 0xffffu) : (u | 0x8000u);
+  }
+  }
+}
+
+__device__ __forceinline__ void store_key32(void *p, int64_t i, int dt, uint32_t k) {
+  switch (dt) {
+  case IFM_U8:
+  case IFM_N0F8: ((uint8_t *)p)[i] = (uint8_t)k; break;
+  case IFM_F32: {
+    uint32_t u = (k == 0xffffffffu) ? 0x7fc00000u : ((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+    ((uint32_t *)p)[i] = u;
+    break;
+  }
+  default: {
+    uint32_t nan = (dt == IFM_BF16) ? 0x7fc0u : 0x7e00u;
+    uint32_t u = (k == 0xffffu) ? nan : ((k & 0x8000u) ? (k & 0x7fffu) : (~k & 0xffffu));
+    ((uint16_t *)p)[i] = (uint16_t)u;
+    break;
+  }
+  }
+}
+
+__device__ __forceinline__ uint64_t key64(const void *p, int64_t i) {
+  uint64_t u = (uint64_t)__double_as_longlong(((const double *)p)[i]);
+  if ((u & 0x7fffffffffffffffull) > 0x7ff0000000000000ull) return ~0ull;
+  return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+
+__device__ __forceinline__ void store_key64(void *p, int64_t i, uint64_t k) {
+  uint64_t u = (k == ~0ull) ? 0x7ff8000000000000ull : ((k >> 63) ? (k & 0x7fffffffffffffffull) : ~k);
+  ((uint64_t *)p)[i] = u;
+}
+
+// ---------------------------------------------------------------------------
+// Host side: canonical geometry.  Arrays are permuted so that axis 0 is the
+// contiguous one; kernels see dense column-major 4-D arrays.
+// ---------------------------------------------------------------------------
+struct Dense4 {
+  void *data = nullptr;
+  int dtype = IFM_F32;
+  int64_t n[MAXD] = {1, 1, 1, 1};
+  int64_t count() const { return n[0] * n[1] * n[2] * n[3]; }
+};
+
+inline size_t dtype_size(int dt) {
+  switch (dt) {
+  case IFM_U8:
+  case IFM_N0F8: return 1;
+  case IFM_F32: return 4;
+  case IFM_F64: return 8;
+  default: return 2;
+  }
+}
+
+// Kernel launch wrappers implemented in the .hip files.  All take canonical
+// (memory-order) geometry; `st` is the stream.
+struct FirPass {  // one 1-D factor applied along canonical axis `axis`
+  int axis;
+  int len;
+  int first;
+  float taps[MAXTAPS];
+};
+
+hipError_t launch_pad(const Dense4 &in, Dense4 &out, const int64_t *lo, int style, double fill, hipStream_t st);
+
+// Separable FIR: out region starts at out_first (canonical axes).  `fill` is
+// the Fill value already scaled for this pass (v * prod of earlier tap sums).
+hipError_t launch_fir1d(const Dense4 &in, const int64_t *in_first, Dense4 &out, const int64_t *out_first,
+                        const int64_t *img_n, const FirPass &p, int style, float fill, hipStream_t st);
+hipError_t launch_sep2d(const Dense4 &in, Dense4 &out, const int64_t *out_first, const FirPass &p0,
+                        const FirPass &p1, int style, float fill, hipStream_t st);
+
+struct DenseKernel {
+  int ndim;
+  int n[MAXD];
+  int first[MAXD];
+  std::vector<float> taps;  // column-major
+};
+hipError_t launch_dense_valu(const Dense4 &in, Dense4 &out, const int64_t *out_first, const DenseKernel &k,
+                             const float *d_taps, int style, float fill, hipStream_t st);
+// MFMA banded-Toeplitz path (2-D, axis 0 and 1 extents up to 64).
+bool dense_mfma_supported(const DenseKernel &k, const Dense4 &in);
+hipError_t launch_dense_mfma(const Dense4 &in, Dense4 &out, const int64_t *out_first, const DenseKernel &k,
+                             void *d_work, size_t work_bytes, int style, float fill, hipStream_t st);
+size_t dense_mfma_workspace(const DenseKernel &k);
+
+// IIR: one cascade stage along canonical axis `axis`, FP64 recursion.
+struct IirCoef {
+  double a[3], b[3], scale, M[9];
+};
+hipError_t launch_iir(const Dense4 &in, Dense4 &out, int axis, const IirCoef &c, int isfill, double fill,
+                      hipStream_t st);
+
+// Order filters.
+hipError_t launch_median(const Dense4 &in, Dense4 &out, const int64_t *out_first, const int *wfirst,
+                         const int *wlen, int style, double fill, hipStream_t st);
+// Separable min/max over truncated windows; with fill, the fill key joins any
+// window that leaves the image.  key buffers: two arrays of count() uint32/64.
+hipError_t launch_minmax(const Dense4 &in, Dense4 *omin, Dense4 *omax, const int64_t *out_first,
+                         const int *wfirst, const int *wlen, int isfill, double fill, void *work,
+                         hipStream_t st);
+size_t minmax_workspace(const Dense4 &in, const int64_t *osize);
+
+// FFT correlation (up to 3-D).
+size_t fft_workspace(const int64_t *N, int nd);
+hipError_t launch_fft_filter(const Dense4 &in, Dense4 &out, const int64_t *out_first, const DenseKernel &k,
+                             const int64_t *Npad, int nd, const int64_t *lo, int style, float fill, void *work,
+                             hipStream_t st);
+
+}  // namespace ifm

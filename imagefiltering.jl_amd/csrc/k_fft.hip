@@ -1,0 +1,401 @@
+// k_fft.hip — FFT correlation (gfx950), the Algorithm.FFT() choice of
+// imfilter (src/imfilter.jl:796-866).
+//
+// As in the reference, the image is padded per axis to N = nextprod([2,3],
+// n + K) with the chosen border (src/border.jl:240-248), the kernel is
+// wrapped into an N-periodic array (FFTView), and the result is
+// ifft(fft(A) .* conj(fft(k))) read back at the output axes.  Transforms are
+// complex FP32 Stockham autosort FFTs of radix 2, 3 and 4, one line per
+// workgroup in LDS (N <= 8192); axes other than the contiguous one are
+// transposed to it and back.  Twiddles come from an FP64-computed table.
+#include <math.h>
+
+#include "ifm_internal.h"
+
+namespace ifm {
+
+constexpr int FFT_MAXN = 8192;
+constexpr int FFT_THREADS = 256;
+constexpr int FFT_MAXJ = FFT_MAXN / (2 * FFT_THREADS);  // radix-2 butterflies per thread
+constexpr int FFT_MAXPASS = 16;
+
+struct FftPlan {
+  int N;
+  int npass;
+  int radix[FFT_MAXPASS];
+};
+
+static bool make_plan(int N, FftPlan &p) {
+  p.N = N;
+  p.npass = 0;
+  int m = N;
+  while (m % 3 == 0) { p.radix[p.npass++] = 3; m /= 3; }
+  while (m % 4 == 0) { p.radix[p.npass++] = 4; m /= 4; }
+  while (m % 2 == 0) { p.radix[p.npass++] = 2; m /= 2; }
+  return m == 1 && N <= FFT_MAXN && p.npass <= FFT_MAXPASS;
+}
+
+__device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+struct FftLinesArgs {
+  float2 *data;        // nlines x N, contiguous lines
+  const float2 *tw;    // W_N^m = exp(-2 pi i m / N), m < N
+  int N;
+  int64_t nlines;
+  int sign;            // -1 forward, +1 inverse (unnormalised)
+  int npass;
+  int radix[FFT_MAXPASS];
+};
+
+template <int R>
+__device__ __forceinline__ void fft_pass(float2 *s, const float2 *tw, int N, int Ns, int sign) {
+  const int NR = N / R;
+  constexpr int J = (R == 2) ? FFT_MAXJ : (R == 3 ? (FFT_MAXN / 3 + FFT_THREADS - 1) / FFT_THREADS
+                                                  : FFT_MAXN / (4 * FFT_THREADS));
+  float2 v[J][R];
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int c = 0; c < J; ++c) {
+    const int j = tid + c * FFT_THREADS;
+    if (j < NR) {
+      const int k = j % Ns;
+      const int step = (N / (Ns * R)) * k;  // twiddle exponent unit
+#pragma unroll
+      for (int i = 0; i < R; ++i) {
+        float2 x = s[j + i * NR];
+        if (i > 0) {
+          float2 w = tw[(step * i) % N];
+          if (sign > 0) w.y = -w.y;
+          x = cmulf(x, w);
+        }
+        v[c][i] = x;
+      }
+      // R-point DFT
+      if (R == 2) {
+        float2 a = v[c][0], b = v[c][1];
+        v[c][0] = make_float2(a.x + b.x, a.y + b.y);
+        v[c][1] = make_float2(a.x - b.x, a.y - b.y);
+      } else if (R == 3) {
+        const float c3 = -0.5f, s3 = 0.86602540378443864676f * (float)sign;
+        float2 a = v[c][0], b = v[c][1], d = v[c][2];
+        float2 t = make_float2(b.x + d.x, b.y + d.y);
+        float2 u = make_float2(b.x - d.x, b.y - d.y);
+        v[c][0] = make_float2(a.x + t.x, a.y + t.y);
+        float2 m = make_float2(a.x + c3 * t.x, a.y + c3 * t.y);
+        // -s3 * i * u = (s3 * u.y, -s3 * u.x) with s3 carrying the sign
+        v[c][1] = make_float2(m.x - s3 * u.y, m.y + s3 * u.x);
+        v[c][2] = make_float2(m.x + s3 * u.y, m.y - s3 * u.x);
+      } else {  // R == 4
+        float2 a = v[c][0], b = v[c][1], d = v[c][2], e = v[c][3];
+        float2 t0 = make_float2(a.x + d.x, a.y + d.y), t1 = make_float2(a.x - d.x, a.y - d.y);
+        float2 t2 = make_float2(b.x + e.x, b.y + e.y), t3 = make_float2(b.x - e.x, b.y - e.y);
+        // multiply t3 by sign * i
+        float2 t3i = (sign < 0) ? make_float2(t3.y, -t3.x) : make_float2(-t3.y, t3.x);
+        v[c][0] = make_float2(t0.x + t2.x, t0.y + t2.y);
+        v[c][2] = make_float2(t0.x - t2.x, t0.y - t2.y);
+        v[c][1] = make_float2(t1.x + t3i.x, t1.y + t3i.y);
+        v[c][3] = make_float2(t1.x - t3i.x, t1.y - t3i.y);
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < J; ++c) {
+    const int j = tid + c * FFT_THREADS;
+    if (j < NR) {
+      const int k = j % Ns;
+      const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+      for (int i = 0; i < R; ++i) s[base + i * Ns] = v[c][i];
+    }
+  }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(FFT_THREADS) k_fft_lines(FftLinesArgs a) {
+  extern __shared__ float2 sbuf[];
+  for (int64_t line = blockIdx.x; line < a.nlines; line += gridDim.x) {
+    float2 *g = a.data + line * a.N;
+    for (int i = threadIdx.x; i < a.N; i += FFT_THREADS) sbuf[i] = g[i];
+    __syncthreads();
+    int Ns = 1;
+    for (int p = 0; p < a.npass; ++p) {
+      int R = a.radix[p];
+      if (R == 2) fft_pass<2>(sbuf, a.tw, a.N, Ns, a.sign);
+      else if (R == 3) fft_pass<3>(sbuf, a.tw, a.N, Ns, a.sign);
+      else fft_pass<4>(sbuf, a.tw, a.N, Ns, a.sign);
+      Ns *= R;
+    }
+    for (int i = threadIdx.x; i < a.N; i += FFT_THREADS) g[i] = sbuf[i];
+    __syncthreads();
+  }
+}
+
+// Swap axis 0 with axis `ax` (1 or 2) of a 3-D complex array n0 x n1 x n2.
+struct TransArgs {
+  const float2 *in;
+  float2 *out;
+  int n0, n1, n2;
+  int ax;
+};
+
+__global__ void __launch_bounds__(256) k_transpose(TransArgs a) {
+  __shared__ float2 tile[32][33];
+  const int na = a.ax == 1 ? a.n1 : a.n2;   // extent of the swapped axis
+  const int nm = a.ax == 1 ? a.n2 : a.n1;   // the untouched axis
+  const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int m = blockIdx.z; m < nm; m += gridDim.z) {
+    for (int r = ty; r < 32; r += 8) {
+      int i = i0 + tx, j = j0 + r;  // i along axis 0, j along swapped axis
+      if (i < a.n0 && j < na) {
+        int64_t idx = a.ax == 1 ? (i + (int64_t)a.n0 * (j + (int64_t)a.n1 * m))
+                                : (i + (int64_t)a.n0 * (m + (int64_t)a.n1 * j));
+        tile[r][tx] = a.in[idx];
+      }
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+      int j = j0 + tx, i = i0 + r;
+      if (i < a.n0 && j < na) {
+        // out dims: ax==1 -> n1 x n0 x n2 ; ax==2 -> n2 x n1 x n0
+        int64_t idx = a.ax == 1 ? (j + (int64_t)a.n1 * (i + (int64_t)a.n0 * m))
+                                : (j + (int64_t)a.n2 * (m + (int64_t)a.n1 * i));
+        a.out[idx] = tile[tx][r];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+struct FftFillArgs {
+  const void *in;
+  float2 *x;
+  int n[3];    // image extents
+  int N[3];    // padded extents
+  int lo[3];
+  int style;
+  int dt;
+  float fill;
+  int64_t total;
+};
+
+__global__ void __launch_bounds__(256) k_fft_fill(FftFillArgs a) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < a.total;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    int64_t r = q;
+    int u0 = (int)(r % a.N[0]); r /= a.N[0];
+    int u1 = (int)(r % a.N[1]); r /= a.N[1];
+    int u2 = (int)r;
+    int s0 = bmap(a.style, u0 - a.lo[0], a.n[0]);
+    int s1 = bmap(a.style, u1 - a.lo[1], a.n[1]);
+    int s2 = bmap(a.style, u2 - a.lo[2], a.n[2]);
+    float v = (s0 | s1 | s2) < 0 ? a.fill
+                                 : load_f(a.in, s0 + (int64_t)a.n[0] * (s1 + (int64_t)a.n[1] * s2), a.dt);
+    a.x[q] = make_float2(v, 0.f);
+  }
+}
+
+struct FftKernArgs {
+  const float *taps;
+  float2 *x;
+  int kn[3], kf[3];
+  int N[3];
+  int64_t ktotal;
+};
+
+__global__ void __launch_bounds__(256) k_fft_kernel_scatter(FftKernArgs a) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < a.ktotal;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    int64_t r = q;
+    int j0 = (int)(r % a.kn[0]); r /= a.kn[0];
+    int j1 = (int)(r % a.kn[1]); r /= a.kn[1];
+    int j2 = (int)r;
+    int u0 = imod(j0 + a.kf[0], a.N[0]);
+    int u1 = imod(j1 + a.kf[1], a.N[1]);
+    int u2 = imod(j2 + a.kf[2], a.N[2]);
+    int64_t idx = u0 + (int64_t)a.N[0] * (u1 + (int64_t)a.N[1] * u2);
+    a.x[idx].x = a.taps[q];  // N > K on every axis, so taps never share a cell
+  }
+}
+
+__global__ void __launch_bounds__(256) k_cmul_conj(float2 *x, const float2 *k, int64_t total) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < total;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    float2 a = x[q], b = k[q];
+    x[q] = make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+  }
+}
+
+struct FftExtractArgs {
+  const float2 *x;
+  void *out;
+  int o[3];
+  int off[3];   // out_first + lo
+  int N[3];
+  float scale;
+  int dt;
+  int64_t total;
+};
+
+__global__ void __launch_bounds__(256) k_fft_extract(FftExtractArgs a) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < a.total;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    int64_t r = q;
+    int c0 = (int)(r % a.o[0]); r /= a.o[0];
+    int c1 = (int)(r % a.o[1]); r /= a.o[1];
+    int c2 = (int)r;
+    int64_t idx = (c0 + a.off[0]) + (int64_t)a.N[0] * ((c1 + a.off[1]) + (int64_t)a.N[1] * (c2 + a.off[2]));
+    store_f(a.out, q, a.dt, a.x[idx].x * a.scale);
+  }
+}
+
+static unsigned grid_for(int64_t total) {
+  int64_t b = (total + 255) / 256;
+  return (unsigned)(b > (1 << 20) ? (1 << 20) : (b < 1 ? 1 : b));
+}
+
+// Twiddle tables live in the workspace after the three complex buffers.
+size_t fft_workspace(const int64_t *N, int nd) {
+  int64_t tot = 1;
+  size_t tw = 0;
+  for (int d = 0; d < nd; ++d) {
+    tot *= N[d];
+    tw += (size_t)N[d];
+  }
+  return (size_t)tot * sizeof(float2) * 3 + tw * sizeof(float2) + 256;  // + kernel taps (caller)
+}
+
+static hipError_t fft_all_axes(float2 *x, float2 *tmp, const int *N, int nd, float2 *const *tw, int sign,
+                               hipStream_t st) {
+  int n3[3] = {N[0], nd > 1 ? N[1] : 1, nd > 2 ? N[2] : 1};
+  for (int ax = 0; ax < nd; ++ax) {
+    FftPlan plan;
+    make_plan(N[ax], plan);
+    FftLinesArgs la;
+    la.tw = tw[ax];
+    la.N = N[ax];
+    la.sign = sign;
+    la.npass = plan.npass;
+    for (int p = 0; p < plan.npass; ++p) la.radix[p] = plan.radix[p];
+    int64_t total = (int64_t)n3[0] * n3[1] * n3[2];
+    la.nlines = total / N[ax];
+    size_t lds = sizeof(float2) * (size_t)N[ax];
+    unsigned blocks = (unsigned)(la.nlines < 65536 ? la.nlines : 65536);
+    if (ax == 0) {
+      la.data = x;
+      hipLaunchKernelGGL(k_fft_lines, dim3(blocks), dim3(FFT_THREADS), lds, st, la);
+    } else {
+      TransArgs t;
+      t.in = x;
+      t.out = tmp;
+      t.n0 = n3[0];
+      t.n1 = n3[1];
+      t.n2 = n3[2];
+      t.ax = ax;
+      int na = ax == 1 ? n3[1] : n3[2], nm = ax == 1 ? n3[2] : n3[1];
+      dim3 g((n3[0] + 31) / 32, (na + 31) / 32, nm < 65535 ? nm : 65535);
+      hipLaunchKernelGGL(k_transpose, g, dim3(256), 0, st, t);
+      la.data = tmp;
+      hipLaunchKernelGGL(k_fft_lines, dim3(blocks), dim3(FFT_THREADS), lds, st, la);
+      // transpose back: the swapped array has dims (na, ., n0) in the same pattern
+      TransArgs b;
+      b.in = tmp;
+      b.out = x;
+      if (ax == 1) { b.n0 = n3[1]; b.n1 = n3[0]; b.n2 = n3[2]; }
+      else { b.n0 = n3[2]; b.n1 = n3[1]; b.n2 = n3[0]; }
+      b.ax = ax;
+      int nb = ax == 1 ? b.n1 : b.n2, mb = ax == 1 ? b.n2 : b.n1;
+      dim3 g2((b.n0 + 31) / 32, (nb + 31) / 32, mb < 65535 ? mb : 65535);
+      hipLaunchKernelGGL(k_transpose, g2, dim3(256), 0, st, b);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t launch_fft_filter(const Dense4 &in, Dense4 &out, const int64_t *out_first, const DenseKernel &k,
+                             const int64_t *Npad, int nd, const int64_t *lo, int style, float fill, void *work,
+                             hipStream_t st) {
+  // Axes 0 .. nd-1 are transformed (each N a product of 2s and 3s); further
+  // axes are batch axes with N = n.
+  int N[3] = {(int)Npad[0], (int)Npad[1], (int)Npad[2]};
+  int64_t tot = (int64_t)N[0] * N[1] * N[2];
+  float2 *x = (float2 *)work;
+  float2 *kx = x + tot;
+  float2 *tmp = kx + tot;
+  float2 *twbase = tmp + tot;
+  // twiddle tables (computed in FP64 on the host)
+  float2 *tw[3];
+  size_t ntw = 0;
+  for (int d = 0; d < nd; ++d) ntw += (size_t)N[d];
+  std::vector<float2> h(ntw);
+  float2 *tp = twbase;
+  size_t at = 0;
+  for (int d = 0; d < nd; ++d) {
+    for (int m = 0; m < N[d]; ++m) {
+      double ang = -2.0 * M_PI * (double)m / (double)N[d];
+      h[at + m] = make_float2((float)cos(ang), (float)sin(ang));
+    }
+    tw[d] = tp;
+    tp += N[d];
+    at += N[d];
+  }
+  {
+    hipError_t e = hipMemcpyAsync(twbase, h.data(), sizeof(float2) * ntw, hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return e;
+  }
+  FftFillArgs fa;
+  fa.in = in.data;
+  fa.x = x;
+  for (int d = 0; d < 3; ++d) {
+    fa.n[d] = (int)in.n[d];
+    fa.N[d] = N[d];
+    fa.lo[d] = (int)lo[d];
+  }
+  fa.style = style;
+  fa.dt = in.dtype;
+  fa.fill = fill;
+  fa.total = tot;
+  hipLaunchKernelGGL(k_fft_fill, dim3(grid_for(tot)), dim3(256), 0, st, fa);
+  hipError_t e = hipMemsetAsync(kx, 0, sizeof(float2) * tot, st);
+  if (e != hipSuccess) return e;
+  // kernel taps were uploaded after the twiddles
+  FftKernArgs ka;
+  ka.taps = (const float *)tp;
+  ka.x = kx;
+  ka.ktotal = 1;
+  for (int d = 0; d < 3; ++d) {
+    ka.kn[d] = d < k.ndim ? k.n[d] : 1;
+    ka.kf[d] = d < k.ndim ? k.first[d] : 0;
+    ka.N[d] = N[d];
+    ka.ktotal *= ka.kn[d];
+  }
+  e = hipMemcpyAsync((void *)tp, k.taps.data(), sizeof(float) * k.taps.size(), hipMemcpyHostToDevice, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_fft_kernel_scatter, dim3(grid_for(ka.ktotal)), dim3(256), 0, st, ka);
+  if ((e = fft_all_axes(x, tmp, N, nd, tw, -1, st)) != hipSuccess) return e;
+  if ((e = fft_all_axes(kx, tmp, N, nd, tw, -1, st)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_cmul_conj, dim3(grid_for(tot)), dim3(256), 0, st, x, (const float2 *)kx, tot);
+  if ((e = fft_all_axes(x, tmp, N, nd, tw, +1, st)) != hipSuccess) return e;
+  FftExtractArgs xa;
+  xa.x = x;
+  xa.out = out.data;
+  for (int d = 0; d < 3; ++d) {
+    xa.o[d] = (int)out.n[d];
+    xa.off[d] = (int)(out_first[d] + lo[d]);
+    xa.N[d] = N[d];
+  }
+  xa.scale = (float)(1.0 / (double)tot);
+  xa.dt = out.dtype;
+  xa.total = out.count();
+  hipLaunchKernelGGL(k_fft_extract, dim3(grid_for(xa.total)), dim3(256), 0, st, xa);
+  hipError_t le = hipGetLastError();
+  (void)hipStreamSynchronize(st);  // the host twiddle buffer must outlive its copy
+  return le;
+}
+
+}  // namespace ifm

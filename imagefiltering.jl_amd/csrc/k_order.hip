@@ -1,0 +1,536 @@
+// k_order.hip — mapwindow order filters (gfx950): median, minimum, maximum,
+// extrema.  Bit-exact: all comparisons run on integer order keys (codes for
+// UInt8/N0f8, isless-ordered keys for floats).
+//
+//  * k_median_net   k x k (k = 3, 5, 7) centred windows: every thread runs a
+//                   generated selection network over a tile of outputs; keys
+//                   of 16 bits or fewer are packed two to a register so each
+//                   v_pk_min_u16 / v_pk_max_u16 serves two tiles.
+//  * k_median_any   any window / border (incl. circular, whose edge windows
+//                   wrap inside the window's intersection with the image,
+//                   src/mapwindow.jl:310-316): counting selection.
+//  * k_minmax_pass  one axis of the separable min/max over truncated windows.
+#include "ifm_internal.h"
+
+namespace ifm {
+
+typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t kmin(uint32_t a, uint32_t b) { return a < b ? a : b; }
+__device__ __forceinline__ uint32_t kmax(uint32_t a, uint32_t b) { return a > b ? a : b; }
+__device__ __forceinline__ u16x2 kmin(u16x2 a, u16x2 b) { return __builtin_elementwise_min(a, b); }
+__device__ __forceinline__ u16x2 kmax(u16x2 a, u16x2 b) { return __builtin_elementwise_max(a, b); }
+
+#include "median_nets.h"
+
+template <int K, int TX, int TY>
+struct MedianNet;
+template <> struct MedianNet<3, 4, 2> {
+  template <typename T> static __device__ __forceinline__ void run(const T (&v)[24], T (&o)[8]) { median_net_3_4_2(v, o); }
+};
+template <> struct MedianNet<5, 2, 2> {
+  template <typename T> static __device__ __forceinline__ void run(const T (&v)[36], T (&o)[4]) { median_net_5_2_2(v, o); }
+};
+template <> struct MedianNet<7, 2, 2> {
+  template <typename T> static __device__ __forceinline__ void run(const T (&v)[64], T (&o)[4]) { median_net_7_2_2(v, o); }
+};
+
+struct MedianArgs {
+  const void *in;
+  void *out;
+  int n0, n1;
+  int o0, o1;
+  int f0, f1;
+  int64_t in_plane, out_plane;
+  int nplanes, o2, n2, f2, f3;
+  int style;
+  int dt;
+  uint32_t fillkey;
+};
+
+// Block = 16 x 16 threads; thread tile TX x TY outputs, PACK tiles stacked in
+// rows (PACK = 2 packs two 16-bit keys per register).
+template <int K, int TX, int TY, int PACK>
+__global__ void __launch_bounds__(256) k_median_net(MedianArgs a) {
+  constexpr int H = K / 2;
+  constexpr int BX = 16 * TX;
+  constexpr int BYH = 16 * TY;         // rows per packed half
+  constexpr int BY = BYH * PACK;
+  constexpr int LW = BX + K - 1;
+  constexpr int LH = BY + K - 1;
+  constexpr int VW = TX + K - 1, VH = TY + K - 1;
+  __shared__ uint32_t tile[LH * LW];
+  const int tid = threadIdx.x;
+  const int tx = tid & 15, ty = tid >> 4;
+  const int x0 = a.f0 + blockIdx.x * BX;   // image coordinate of first output column
+  const int y0 = a.f1 + blockIdx.y * BY;
+  for (int plane = blockIdx.z; plane < a.nplanes; plane += gridDim.z) {
+    const int p2 = plane % a.o2, p3 = plane / a.o2;
+    const int64_t ibase = ((int64_t)(p2 + a.f2) + (int64_t)a.n2 * (p3 + a.f3)) * a.in_plane;
+    for (int i = tid; i < LH * LW; i += 256) {
+      int r = i / LW, c = i - r * LW;
+      int sy = bmap(a.style, y0 - H + r, a.n1);
+      int sx = bmap(a.style, x0 - H + c, a.n0);
+      tile[i] = (sy < 0 || sx < 0) ? a.fillkey : key32(a.in, ibase + (int64_t)sy * a.n0 + sx, a.dt);
+    }
+    __syncthreads();
+    const int64_t obase = (int64_t)plane * a.out_plane;
+    const int cx = tx * TX, cy = ty * TY;
+    if constexpr (PACK == 2) {
+      u16x2 v[VW * VH];
+#pragma unroll
+      for (int r = 0; r < VH; ++r)
+#pragma unroll
+        for (int c = 0; c < VW; ++c) {
+          u16x2 p;
+          p.x = (uint16_t)tile[(cy + r) * LW + cx + c];
+          p.y = (uint16_t)tile[(cy + BYH + r) * LW + cx + c];
+          v[c + VW * r] = p;
+        }
+      u16x2 o[TX * TY];
+      MedianNet<K, TX, TY>::run(v, o);
+#pragma unroll
+      for (int r = 0; r < TY; ++r)
+#pragma unroll
+        for (int c = 0; c < TX; ++c) {
+          int ox = blockIdx.x * BX + cx + c;
+          int oy = blockIdx.y * BY + cy + r;
+          if (ox < a.o0) {
+            if (oy < a.o1) store_key32(a.out, obase + (int64_t)oy * a.o0 + ox, a.dt, o[c + TX * r].x);
+            if (oy + BYH < a.o1) store_key32(a.out, obase + (int64_t)(oy + BYH) * a.o0 + ox, a.dt, o[c + TX * r].y);
+          }
+        }
+    } else {
+      uint32_t v[VW * VH];
+#pragma unroll
+      for (int r = 0; r < VH; ++r)
+#pragma unroll
+        for (int c = 0; c < VW; ++c) v[c + VW * r] = tile[(cy + r) * LW + cx + c];
+      uint32_t o[TX * TY];
+      MedianNet<K, TX, TY>::run(v, o);
+#pragma unroll
+      for (int r = 0; r < TY; ++r)
+#pragma unroll
+        for (int c = 0; c < TX; ++c) {
+          int ox = blockIdx.x * BX + cx + c;
+          int oy = blockIdx.y * BY + cy + r;
+          if (ox < a.o0 && oy < a.o1) store_key32(a.out, obase + (int64_t)oy * a.o0 + ox, a.dt, o[c + TX * r]);
+        }
+    }
+    __syncthreads();
+  }
+}
+
+// Generic window median: the thread enumerates its window twice (candidate,
+// then rank count), computing each mapped position on the fly.
+struct MedianAnyArgs {
+  const void *in;
+  void *out;
+  int n[MAXD];
+  int o[MAXD];
+  int of[MAXD];
+  int wf[MAXD];
+  int wl[MAXD];
+  int style;
+  int dt;
+  double fill;
+  int64_t total;
+  int L;
+};
+
+template <typename KT>
+__device__ __forceinline__ KT window_key(const MedianAnyArgs &a, const int *q, const int *ilo, const int *ilen, int j) {
+  int s[MAXD];
+  bool isfill = false;
+#pragma unroll
+  for (int d = 0; d < MAXD; ++d) {
+    int jd = j % a.wl[d];
+    j /= a.wl[d];
+    int x = q[d] + a.wf[d] + jd;
+    if ((unsigned)x < (unsigned)a.n[d]) {
+      s[d] = x;
+    } else if (a.style == IFM_FILL || ilen[d] <= 0) {
+      isfill = true;
+      s[d] = 0;
+    } else {
+      s[d] = ilo[d] + bmap(a.style, x - ilo[d], ilen[d]);
+    }
+  }
+  if (isfill) {
+    if constexpr (sizeof(KT) == 8) {
+      double f = a.fill;
+      uint64_t u = (uint64_t)__double_as_longlong(f);
+      if ((u & 0x7fffffffffffffffull) > 0x7ff0000000000000ull) return ~0ull;
+      return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+    } else {
+      // fill already converted to the eltype; reuse the key function on a local
+      float ff = (float)a.fill;
+      uint32_t kk;
+      if (a.dt == IFM_U8 || a.dt == IFM_N0F8) kk = (uint32_t)(int)a.fill;
+      else if (a.dt == IFM_F32) kk = key32(&ff, 0, IFM_F32);
+      else if (a.dt == IFM_BF16) { uint16_t b = f32_to_bf16(ff); kk = key32(&b, 0, IFM_BF16); }
+      else { __half h = __float2half_rn(ff); kk = key32(&h, 0, IFM_F16); }
+      return kk;
+    }
+  }
+  int64_t idx = s[0] + (int64_t)a.n[0] * (s[1] + (int64_t)a.n[1] * (s[2] + (int64_t)a.n[2] * s[3]));
+  if constexpr (sizeof(KT) == 8) return key64(a.in, idx);
+  else return key32(a.in, idx, a.dt);
+}
+
+template <typename KT>
+__global__ void __launch_bounds__(256) k_median_any(MedianAnyArgs a) {
+  const int rank = (a.L + 1) / 2 - 1;
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < a.total;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    int64_t r = p;
+    int q[MAXD], ilo[MAXD], ilen[MAXD];
+#pragma unroll
+    for (int d = 0; d < MAXD; ++d) {
+      q[d] = (int)(r % a.o[d]) + a.of[d];
+      r /= a.o[d];
+      int lo = q[d] + a.wf[d], hi = q[d] + a.wf[d] + a.wl[d] - 1;
+      ilo[d] = lo < 0 ? 0 : lo;
+      int h2 = hi > a.n[d] - 1 ? a.n[d] - 1 : hi;
+      ilen[d] = h2 - ilo[d] + 1;
+    }
+    KT result = 0;
+    for (int i = 0; i < a.L; ++i) {
+      KT ki = window_key<KT>(a, q, ilo, ilen, i);
+      int less = 0, eq = 0;
+      for (int j = 0; j < a.L; ++j) {
+        KT kj = window_key<KT>(a, q, ilo, ilen, j);
+        less += kj < ki;
+        eq += kj == ki;
+      }
+      if (less <= rank && rank < less + eq) {
+        result = ki;
+        break;
+      }
+    }
+    if constexpr (sizeof(KT) == 8) store_key64(a.out, p, result);
+    else store_key32(a.out, p, a.dt, result);
+  }
+}
+
+static uint32_t fill_key32(int dt, double fill) {
+  // host twin of key32 for the fill value
+  switch (dt) {
+  case IFM_U8:
+  case IFM_N0F8: return (uint32_t)(int)fill;
+  case IFM_F32: {
+    float f = (float)fill;
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return 0xffffffffu;
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+  }
+  default: {
+    uint32_t u;
+    float f = (float)fill;
+    if (dt == IFM_BF16) {
+      uint32_t b;
+      memcpy(&b, &f, 4);
+      u = (b + 0x7fffu + ((b >> 16) & 1u)) >> 16;
+    } else {
+      __half h = __float2half_rn(f);
+      uint16_t hb;
+      memcpy(&hb, &h, 2);
+      u = hb;
+    }
+    uint32_t expmask = (dt == IFM_BF16) ? 0x7f80u : 0x7c00u;
+    if ((u & 0x7fffu) > expmask) return 0xffffu;
+    return (u & 0x8000u) ? (~u & 0xffffu) : (u | 0x8000u);
+  }
+  }
+}
+
+template <int K, int TX, int TY, int PACK>
+static void launch_net_on(const MedianArgs &a, hipStream_t st) {
+  constexpr int BX = 16 * TX, BY = 16 * TY * PACK;
+  dim3 grid((a.o0 + BX - 1) / BX, (a.o1 + BY - 1) / BY, a.nplanes < 65535 ? a.nplanes : 65535);
+  hipLaunchKernelGGL((k_median_net<K, TX, TY, PACK>), grid, dim3(256), 0, st, a);
+}
+
+hipError_t launch_median(const Dense4 &in, Dense4 &out, const int64_t *out_first, const int *wfirst,
+                         const int *wlen, int style, double fill, hipStream_t st) {
+  if (out.count() == 0) return hipSuccess;
+  // The tiled network path needs a centred odd square 2-D window and an
+  // image-relative border map (which equals mapwindow's window-relative one
+  // for centred windows except for circular).
+  int k = wlen[0];
+  bool square = (k == 3 || k == 5 || k == 7) && wlen[1] == k && wfirst[0] == -(k / 2) && wfirst[1] == -(k / 2) &&
+                wlen[2] == 1 && wlen[3] == 1 && wfirst[2] == 0 && wfirst[3] == 0;
+  bool styleok = style != IFM_CIRCULAR;
+  bool small = in.n[0] >= k && in.n[1] >= k;  // mirrored maps stay within the window
+  if (square && styleok && small && in.dtype != IFM_F64) {
+    MedianArgs a;
+    a.in = in.data;
+    a.out = out.data;
+    a.n0 = (int)in.n[0];
+    a.n1 = (int)in.n[1];
+    a.o0 = (int)out.n[0];
+    a.o1 = (int)out.n[1];
+    a.f0 = (int)out_first[0];
+    a.f1 = (int)out_first[1];
+    a.in_plane = in.n[0] * in.n[1];
+    a.out_plane = out.n[0] * out.n[1];
+    a.nplanes = (int)(out.n[2] * out.n[3]);
+    a.o2 = (int)out.n[2];
+    a.n2 = (int)in.n[2];
+    a.f2 = (int)out_first[2];
+    a.f3 = (int)out_first[3];
+    a.style = style;
+    a.dt = in.dtype;
+    a.fillkey = fill_key32(in.dtype, fill);
+    bool pack = in.dtype != IFM_F32;
+    if (k == 3) {
+      if (pack) launch_net_on<3, 4, 2, 2>(a, st); else launch_net_on<3, 4, 2, 1>(a, st);
+    } else if (k == 5) {
+      if (pack) launch_net_on<5, 2, 2, 2>(a, st); else launch_net_on<5, 2, 2, 1>(a, st);
+    } else {
+      if (pack) launch_net_on<7, 2, 2, 2>(a, st); else launch_net_on<7, 2, 2, 1>(a, st);
+    }
+    return hipGetLastError();
+  }
+  MedianAnyArgs a;
+  a.in = in.data;
+  a.out = out.data;
+  a.L = 1;
+  for (int d = 0; d < MAXD; ++d) {
+    a.n[d] = (int)in.n[d];
+    a.o[d] = (int)out.n[d];
+    a.of[d] = (int)out_first[d];
+    a.wf[d] = wfirst[d];
+    a.wl[d] = wlen[d];
+    a.L *= wlen[d];
+  }
+  a.style = style;
+  a.dt = in.dtype;
+  a.fill = fill;
+  a.total = out.count();
+  int64_t blocks = (a.total + 255) / 256;
+  if (blocks > 1 << 20) blocks = 1 << 20;
+  if (in.dtype == IFM_F64)
+    hipLaunchKernelGGL(k_median_any<uint64_t>, dim3((unsigned)blocks), dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL(k_median_any<uint32_t>, dim3((unsigned)blocks), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// min / max over truncated windows.  Julia's min/max on floats propagate NaN
+// and order -0.0 below +0.0: the min pipeline maps NaN to key 0 (smallest),
+// the max pipeline to the all-ones key (largest).
+// ---------------------------------------------------------------------------
+template <typename KT>
+__device__ __forceinline__ KT minkey_of(KT k, int dt) {
+  if constexpr (sizeof(KT) == 8) return k == ~0ull ? 0ull : k;
+  else {
+    if (dt == IFM_F32) return k == 0xffffffffu ? 0u : k;
+    if (dt == IFM_BF16 || dt == IFM_F16) return k == 0xffffu ? 0u : k;
+    return k;
+  }
+}
+template <typename KT>
+__device__ __forceinline__ KT minkey_back(KT k, int dt) {
+  if constexpr (sizeof(KT) == 8) return k == 0ull ? ~0ull : k;
+  else {
+    if (dt == IFM_F32) return k == 0u ? 0xffffffffu : k;
+    if (dt == IFM_BF16 || dt == IFM_F16) return k == 0u ? 0xffffu : k;
+    return k;
+  }
+}
+template <typename KT> __device__ __forceinline__ KT key_top(int dt) {
+  if constexpr (sizeof(KT) == 8) return ~0ull;
+  else return (dt == IFM_U8 || dt == IFM_N0F8) ? 0xffu : ((dt == IFM_F32) ? 0xffffffffu : 0xffffu);
+}
+
+template <typename KT>
+__device__ __forceinline__ KT load_key(const void *p, int64_t i, int dt) {
+  if constexpr (sizeof(KT) == 8) return key64(p, i);
+  else return key32(p, i, dt);
+}
+template <typename KT>
+__device__ __forceinline__ void put_key(void *p, int64_t i, int dt, KT k) {
+  if constexpr (sizeof(KT) == 8) store_key64(p, i, k);
+  else store_key32(p, i, dt, k);
+}
+
+// One separable pass along canonical axis `axis` over key arrays (first pass
+// reads the image, later passes read the previous pass's keys).  The last
+// pass writes the output eltype and merges the fill key where a window leaves
+// the image on any axis.
+struct MinmaxPassArgs {
+  const void *src_img;   // first pass: image; else nullptr
+  const void *src_min;   // key buffers of the previous pass
+  const void *src_max;
+  void *dst_min;         // keys (intermediate) or output (last pass)
+  void *dst_max;
+  int in_n[MAXD];        // extents of the source
+  int out_n[MAXD];
+  int shift[MAXD];       // out coord + shift = source coord (non-filtered axes)
+  int img_n[MAXD];
+  int of[MAXD];          // out_first (image coordinates of out index 0)
+  int wf[MAXD], wl[MAXD];
+  int axis;
+  int first_pass, last_pass;
+  int dt;
+  int isfill;
+  int64_t total;
+};
+
+template <typename KT>
+__global__ void __launch_bounds__(256) k_minmax_pass(MinmaxPassArgs a, KT fillkey) {
+  const KT top = key_top<KT>(a.dt);
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < a.total;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    int64_t r = q;
+    int o[MAXD];
+#pragma unroll
+    for (int d = 0; d < MAXD; ++d) {
+      o[d] = (int)(r % a.out_n[d]);
+      r /= a.out_n[d];
+    }
+    int64_t base = 0, stride = 1, astride = 1;
+#pragma unroll
+    for (int d = 0; d < MAXD; ++d) {
+      if (d == a.axis) astride = stride;
+      else base += (int64_t)(o[d] + a.shift[d]) * stride;
+      stride *= a.in_n[d];
+    }
+    const int n = a.in_n[a.axis];
+    const int x = o[a.axis] + a.of[a.axis] + a.wf[a.axis];
+    int lo = x < 0 ? 0 : x;
+    int hi = x + a.wl[a.axis] - 1;
+    if (hi > n - 1) hi = n - 1;
+    KT mn = top, mx = 0;
+    for (int s = lo; s <= hi; ++s) {
+      int64_t idx = base + (int64_t)s * astride;
+      KT kn, kx;
+      if (a.first_pass) {
+        KT k = load_key<KT>(a.src_img, idx, a.dt);
+        kn = minkey_of<KT>(k, a.dt);
+        kx = k;
+      } else {
+        kn = a.src_min ? ((const KT *)a.src_min)[idx] : top;
+        kx = a.src_max ? ((const KT *)a.src_max)[idx] : 0;
+      }
+      mn = kn < mn ? kn : mn;
+      mx = kx > mx ? kx : mx;
+    }
+    if (a.last_pass) {
+      if (a.isfill) {
+        bool outside = false;
+#pragma unroll
+        for (int d = 0; d < MAXD; ++d) {
+          int xd = (d == a.axis ? o[d] : o[d]) + a.of[d] + a.wf[d];
+          if (xd < 0 || xd + a.wl[d] - 1 > a.img_n[d] - 1) outside = true;
+        }
+        if (outside) {
+          KT fn = minkey_of<KT>(fillkey, a.dt);
+          mn = fn < mn ? fn : mn;
+          mx = fillkey > mx ? fillkey : mx;
+        }
+      }
+      if (a.dst_min) put_key<KT>(a.dst_min, q, a.dt, minkey_back<KT>(mn, a.dt));
+      if (a.dst_max) put_key<KT>(a.dst_max, q, a.dt, mx);
+    } else {
+      if (a.dst_min) ((KT *)a.dst_min)[q] = mn;
+      if (a.dst_max) ((KT *)a.dst_max)[q] = mx;
+    }
+  }
+}
+
+size_t minmax_workspace(const Dense4 &in, const int64_t *osize) {
+  // two ping-pong key buffers per extreme, sized for the largest intermediate
+  size_t ks = in.dtype == IFM_F64 ? 8 : 4;
+  int64_t c = 1;
+  for (int d = 0; d < MAXD; ++d) c *= (in.n[d] > osize[d] ? in.n[d] : osize[d]);
+  return 4 * (size_t)c * ks;
+}
+
+hipError_t launch_minmax(const Dense4 &in, Dense4 *omin, Dense4 *omax, const int64_t *out_first,
+                         const int *wfirst, const int *wlen, int isfill, double fill, void *work,
+                         hipStream_t st) {
+  Dense4 &ref = omin ? *omin : *omax;
+  if (ref.count() == 0) return hipSuccess;
+  bool k64 = in.dtype == IFM_F64;
+  size_t ks = k64 ? 8 : 4;
+  int64_t osz[MAXD];
+  for (int d = 0; d < MAXD; ++d) osz[d] = ref.n[d];
+  int64_t cap = 1;
+  for (int d = 0; d < MAXD; ++d) cap *= (in.n[d] > osz[d] ? in.n[d] : osz[d]);
+  char *w = (char *)work;
+  void *bufmin[2] = {w, w + cap * ks};
+  void *bufmax[2] = {w + 2 * cap * ks, w + 3 * cap * ks};
+  // axes with a window longer than 1 (or offset) are filtered; at least one pass
+  int axes[MAXD], na = 0;
+  for (int d = 0; d < MAXD; ++d)
+    if (wlen[d] != 1 || wfirst[d] != 0 || osz[d] != in.n[d] || out_first[d] != 0) axes[na++] = d;
+  if (na == 0) axes[na++] = 0;
+  // current source extents / offsets (image coordinates)
+  int64_t cur_n[MAXD], cur_first[MAXD];
+  for (int d = 0; d < MAXD; ++d) { cur_n[d] = in.n[d]; cur_first[d] = 0; }
+  const void *smin = nullptr, *smax = nullptr;
+  uint64_t fk64 = 0;
+  uint32_t fk32 = fill_key32(in.dtype, fill);
+  if (k64) {
+    uint64_t u;
+    memcpy(&u, &fill, 8);
+    fk64 = ((u & 0x7fffffffffffffffull) > 0x7ff0000000000000ull) ? ~0ull
+           : ((u >> 63) ? ~u : (u | 0x8000000000000000ull));
+  }
+  for (int p = 0; p < na; ++p) {
+    int ax = axes[p];
+    MinmaxPassArgs a;
+    a.first_pass = p == 0;
+    a.last_pass = p == na - 1;
+    a.src_img = in.data;
+    a.src_min = smin;
+    a.src_max = smax;
+    int64_t nn[MAXD], nf[MAXD];
+    for (int d = 0; d < MAXD; ++d) {
+      // after filtering axis d the buffer covers the output range on d
+      bool done = false;
+      for (int q = 0; q <= p; ++q) done |= axes[q] == d;
+      nn[d] = done ? osz[d] : in.n[d];
+      nf[d] = done ? out_first[d] : 0;
+    }
+    for (int d = 0; d < MAXD; ++d) {
+      a.in_n[d] = (int)cur_n[d];
+      a.out_n[d] = (int)nn[d];
+      a.shift[d] = (int)(nf[d] - cur_first[d]);
+      a.img_n[d] = (int)in.n[d];
+      a.of[d] = (int)nf[d];
+      a.wf[d] = wfirst[d];
+      a.wl[d] = wlen[d];
+    }
+    a.shift[ax] = 0;
+    a.axis = ax;
+    a.dt = in.dtype;
+    a.isfill = isfill;
+    a.total = nn[0] * nn[1] * nn[2] * nn[3];
+    if (a.last_pass) {
+      a.dst_min = omin ? omin->data : nullptr;
+      a.dst_max = omax ? omax->data : nullptr;
+    } else {
+      a.dst_min = omin ? bufmin[p & 1] : nullptr;
+      a.dst_max = omax ? bufmax[p & 1] : nullptr;
+    }
+    int64_t blocks = (a.total + 255) / 256;
+    if (blocks > 1 << 20) blocks = 1 << 20;
+    if (k64)
+      hipLaunchKernelGGL(k_minmax_pass<uint64_t>, dim3((unsigned)blocks), dim3(256), 0, st, a, fk64);
+    else
+      hipLaunchKernelGGL(k_minmax_pass<uint32_t>, dim3((unsigned)blocks), dim3(256), 0, st, a, fk32);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    smin = a.dst_min;
+    smax = a.dst_max;
+    for (int d = 0; d < MAXD; ++d) { cur_n[d] = nn[d]; cur_first[d] = nf[d]; }
+  }
+  return hipSuccess;
+}
+
+}  // namespace ifm

@@ -1,0 +1,141 @@
+"""ctypes binding of libifm.so (the C ABI in include/ifm.h).
+
+The library is built in-tree by ``__graft_entry__.build()`` (or ``make -C
+imagefiltering.jl_amd/csrc``).  There is no fallback: every filtering call
+of this package runs through libifm on the GPU, and a missing library or GPU
+raises.
+"""
+import ctypes as C
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.normpath(os.path.join(_HERE, "..", "libifm.so"))
+
+MAXD = 4
+
+# ifm_status
+OK, E_ARG, E_DIM, E_NOMEM, E_DEVICE, E_UNSUPPORTED, E_BUDGET = range(7)
+# ifm_dtype
+U8, N0F8, F32, F64, BF16, F16 = range(6)
+# ifm_style
+REPLICATE, CIRCULAR, SYMMETRIC, REFLECT, FILL = range(5)
+# ifm_precision
+PREC_AUTO, PREC_FP32, PREC_BF16X3 = range(3)
+
+i64x4 = C.c_int64 * MAXD
+
+
+class IfmArray(C.Structure):
+    _fields_ = [("ndim", C.c_int32), ("dtype", C.c_int32), ("size", i64x4), ("stride", i64x4),
+                ("data", C.c_void_p), ("device", C.c_int32), ("reserved", C.c_int32)]
+
+
+class IfmBorder(C.Structure):
+    _fields_ = [("style", C.c_int32), ("reserved", C.c_int32), ("fill", C.c_double),
+                ("lo", i64x4), ("hi", i64x4)]
+
+
+class IfmFactor(C.Structure):
+    _fields_ = [("axis", C.c_int32), ("len", C.c_int32), ("first", C.c_int64),
+                ("taps", C.POINTER(C.c_double))]
+
+
+class IfmKernel(C.Structure):
+    _fields_ = [("ndim", C.c_int32), ("reserved", C.c_int32), ("size", i64x4), ("first", i64x4),
+                ("taps", C.POINTER(C.c_double))]
+
+
+class IfmIir(C.Structure):
+    _fields_ = [("a", C.c_double * 3), ("b", C.c_double * 3), ("scale", C.c_double),
+                ("M", C.c_double * 9), ("axis", C.c_int32), ("reserved", C.c_int32)]
+
+
+EXPORTS = [
+    "ifm_ctx_create", "ifm_ctx_destroy", "ifm_ctx_set_stream", "ifm_ctx_synchronize", "ifm_last_error",
+    "ifm_version", "ifm_padarray", "ifm_imfilter_fir", "ifm_imfilter_dense", "ifm_imfilter_fft",
+    "ifm_imfilter_iir", "ifm_mapwindow_median", "ifm_mapwindow_minmax", "ifm_extrema_filter",
+    "ifm_device_alloc", "ifm_device_free", "ifm_memcpy_h2d", "ifm_memcpy_d2h", "ifm_memset_d",
+    "ifm_timer_start", "ifm_timer_stop", "ifm_device_count",
+]
+
+_lib = None
+_ctx = None
+_lock = threading.Lock()
+
+
+def load(path=LIBPATH):
+    """Load libifm.so (no GPU needed); raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError("libifm.so is not built at %s (run __graft_entry__.build())" % path)
+    L = C.CDLL(path)
+    P = C.POINTER
+    vp = C.c_void_p
+    L.ifm_ctx_create.argtypes = [C.c_int, vp, P(vp)]
+    L.ifm_ctx_destroy.argtypes = [vp]
+    L.ifm_ctx_set_stream.argtypes = [vp, vp]
+    L.ifm_ctx_synchronize.argtypes = [vp]
+    L.ifm_last_error.restype = C.c_char_p
+    L.ifm_last_error.argtypes = [vp]
+    L.ifm_version.restype = C.c_char_p
+    L.ifm_padarray.argtypes = [vp, P(IfmArray), P(IfmArray), P(IfmBorder)]
+    L.ifm_imfilter_fir.argtypes = [vp, P(IfmArray), P(IfmArray), P(C.c_int64), P(IfmFactor), C.c_int,
+                                   P(IfmBorder)]
+    L.ifm_imfilter_dense.argtypes = [vp, P(IfmArray), P(IfmArray), P(C.c_int64), P(IfmKernel), P(IfmBorder),
+                                     C.c_int]
+    L.ifm_imfilter_fft.argtypes = [vp, P(IfmArray), P(IfmArray), P(C.c_int64), P(IfmKernel), P(IfmBorder)]
+    L.ifm_imfilter_iir.argtypes = [vp, P(IfmArray), P(IfmArray), P(IfmIir), C.c_int, P(IfmBorder)]
+    L.ifm_mapwindow_median.argtypes = [vp, P(IfmArray), P(IfmArray), P(C.c_int64), P(C.c_int64), P(C.c_int64),
+                                       P(IfmBorder)]
+    L.ifm_mapwindow_minmax.argtypes = [vp, P(IfmArray), P(IfmArray), P(IfmArray), P(C.c_int64), P(C.c_int64),
+                                       P(C.c_int64), P(IfmBorder)]
+    L.ifm_extrema_filter.argtypes = [vp, P(IfmArray), P(IfmArray), P(IfmArray), P(C.c_int64)]
+    L.ifm_device_alloc.argtypes = [vp, C.c_uint64, P(vp)]
+    L.ifm_device_free.argtypes = [vp, vp]
+    L.ifm_memcpy_h2d.argtypes = [vp, vp, vp, C.c_uint64]
+    L.ifm_memcpy_d2h.argtypes = [vp, vp, vp, C.c_uint64]
+    L.ifm_memset_d.argtypes = [vp, vp, C.c_int, C.c_uint64]
+    L.ifm_timer_start.argtypes = [vp]
+    L.ifm_timer_stop.argtypes = [vp, P(C.c_float)]
+    L.ifm_device_count.argtypes = [P(C.c_int)]
+    _lib = L
+    return L
+
+
+def ctx():
+    """The process-wide context on device 0 (created on first use)."""
+    global _ctx
+    with _lock:
+        if _ctx is None:
+            L = load()
+            h = C.c_void_p()
+            rc = L.ifm_ctx_create(0, None, C.byref(h))
+            if rc != OK:
+                raise RuntimeError("libifm: no usable GPU (ifm_ctx_create returned %d)" % rc)
+            _ctx = h
+        return _ctx
+
+
+def check(rc):
+    """Map an ifm_status to the reference's exception types."""
+    if rc == OK:
+        return
+    from .errors import ArgumentError, DimensionMismatch
+    msg = load().ifm_last_error(ctx()).decode(errors="replace")
+    if rc == E_ARG:
+        raise ArgumentError(msg)
+    if rc == E_DIM:
+        raise DimensionMismatch(msg)
+    if rc == E_UNSUPPORTED:
+        raise NotImplementedError(msg)
+    if rc == E_NOMEM:
+        raise MemoryError(msg)
+    raise RuntimeError("libifm error %d: %s" % (rc, msg))
+
+
+def i64(vals, fill=0):
+    v = list(vals) + [fill] * (MAXD - len(vals))
+    return i64x4(*[int(x) for x in v[:MAXD]])

@@ -1,0 +1,141 @@
+"""Element types of the filtering path and Julia's promotion rules for them.
+
+numpy dtypes stand for Julia's Float16/32/64 and UInt8.  N0f8 (FixedPointNumbers,
+UInt8 storage read as code/255) and BFloat16 (UInt16 storage) are marked with
+ndarray subclasses, since numpy has no such scalar types.
+"""
+import numpy as np
+
+from . import _lib
+
+
+class Eltype:
+    def __init__(self, name, code, storage, kind):
+        self.name, self.code, self.storage, self.kind = name, code, np.dtype(storage), kind
+
+    def __repr__(self):
+        return self.name
+
+    @property
+    def itemsize(self):
+        return self.storage.itemsize
+
+
+UInt8 = Eltype("UInt8", _lib.U8, np.uint8, "int")
+N0f8 = Eltype("N0f8", _lib.N0F8, np.uint8, "fixed")
+Float16 = Eltype("Float16", _lib.F16, np.float16, "float")
+BFloat16 = Eltype("BFloat16", _lib.BF16, np.uint16, "float")
+Float32 = Eltype("Float32", _lib.F32, np.float32, "float")
+Float64 = Eltype("Float64", _lib.F64, np.float64, "float")
+Int64 = Eltype("Int64", None, np.int64, "int")   # kernels / promotion only
+Bool = Eltype("Bool", None, np.bool_, "int")
+
+_BY_DTYPE = {np.dtype(np.uint8): UInt8, np.dtype(np.float16): Float16, np.dtype(np.float32): Float32,
+             np.dtype(np.float64): Float64, np.dtype(np.int64): Int64, np.dtype(np.int32): Int64,
+             np.dtype(np.int16): Int64, np.dtype(np.int8): Int64, np.dtype(np.uint16): Int64,
+             np.dtype(np.uint32): Int64, np.dtype(np.bool_): Bool}
+
+
+class N0f8Array(np.ndarray):
+    """UInt8 codes read as N0f8 (code/255) by linear filters."""
+
+
+class BF16Array(np.ndarray):
+    """UInt16 bit patterns of BFloat16 values."""
+
+
+def n0f8(codes):
+    """Tag a uint8 array as N0f8 (like reinterpret(N0f8, codes))."""
+    return np.asarray(codes, dtype=np.uint8).view(N0f8Array)
+
+
+def bf16(values):
+    """Round float values to BFloat16 (nearest even) and tag the bits."""
+    a = np.asarray(values, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    nan = np.isnan(a)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+    r[nan] = ((u[nan] >> 16) | 0x40).astype(np.uint16)
+    return r.view(BF16Array)
+
+
+def bf16_to_float32(bits):
+    return (np.asarray(bits, dtype=np.uint16).astype(np.uint32) << 16).view(np.float32)
+
+
+def eltype_of(x):
+    """Element type of an array (numpy, tagged numpy, DeviceArray) or kernel."""
+    et = getattr(x, "eltype", None)
+    if isinstance(et, Eltype):
+        return et
+    if isinstance(x, N0f8Array):
+        return N0f8
+    if isinstance(x, BF16Array):
+        return BFloat16
+    dt = np.asarray(x).dtype
+    if dt in _BY_DTYPE:
+        return _BY_DTYPE[dt]
+    raise TypeError("unsupported element type %s" % dt)
+
+
+def as_eltype(T):
+    """Accept an Eltype, a numpy dtype or scalar type (e.g. np.float32)."""
+    if isinstance(T, Eltype):
+        return T
+    try:
+        dt = np.dtype(T)
+    except TypeError:
+        raise TypeError("not an element type: %r" % (T,))
+    if dt in _BY_DTYPE:
+        return _BY_DTYPE[dt]
+    raise TypeError("unsupported element type %s" % dt)
+
+
+def is_eltype_spec(x):
+    if isinstance(x, Eltype):
+        return True
+    if isinstance(x, type) and issubclass(x, np.generic):
+        return True
+    return isinstance(x, np.dtype)
+
+
+_FLOATRANK = {"Float16": 1, "BFloat16": 1, "Float32": 2, "Float64": 3}
+
+
+def promote_float(a, b):
+    """promote_type for two float eltypes (BFloat16 with Float16 -> Float32)."""
+    if a is b:
+        return a
+    ra, rb = _FLOATRANK[a.name], _FLOATRANK[b.name]
+    if ra == rb:
+        return Float32
+    return a if ra > rb else b
+
+
+def mul_type(S, T):
+    """typeof(zero(S) * zero(T)) for a pixel eltype S and a tap eltype T
+    (src/imfilter.jl:1131-1134)."""
+    if T.kind == "float":
+        if S.kind == "float":
+            return promote_float(S, T)
+        if S is N0f8:  # FixedPointNumbers promotes N0f8 with a float to floattype(N0f8)=Float32 first
+            return promote_float(Float32, T)
+        return T       # UInt8 * Float -> Float
+    # integer / Bool taps
+    if S.kind == "float":
+        return S
+    if S is N0f8:
+        return Float32
+    return Int64
+
+
+def promote(a, b):
+    if a is b:
+        return a
+    if a.kind == "float" and b.kind == "float":
+        return promote_float(a, b)
+    if a.kind == "float":
+        return a
+    if b.kind == "float":
+        return b
+    return Int64

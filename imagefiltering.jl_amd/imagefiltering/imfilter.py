@@ -1,0 +1,509 @@
+"""imfilter / imfilter! (src/imfilter.jl:1-254) on the MI355X.
+
+The dispatch steps follow the reference: (1) output eltype (filter_type),
+(2) kernel factoring (factorkernel, SVD rank test), (3) default border
+"replicate", (4) output allocation (Inner -> interior axes), (5) algorithm
+choice (filter_algorithm) and resource.  Every algorithm then runs on the GPU
+through libifm; there is no CPU compute path.  Julia's `imfilter!` is
+`imfilter_` here.
+"""
+import numpy as np
+
+from . import _lib
+from .border import (STYLES, AbstractBorder, Fill, Inner, NA, NoPad, Pad, borderinstance, calculate_padding,
+                     convert_fill)
+from .device import DeviceArray, empty_like, host_dense, ifm_array
+from .eltypes import (BFloat16, Bool, Float16, Float32, Float64, Int64, N0f8, UInt8, as_eltype, eltype_of,
+                      is_eltype_spec, mul_type, promote)
+from .errors import ArgumentError, DimensionMismatch, MethodError
+from .kernel import Laplacian
+from .kernelfactors import ReshapedOneD, TriggsSdika
+from .offsetarrays import OffsetArray, kernelshift
+
+
+# ---- Algorithm tags and resources (src/ImageFiltering.jl:52-68) -------------------
+class Alg:
+    def __eq__(self, other):
+        return type(self) is type(other)
+
+    def __hash__(self):
+        return hash(type(self).__name__)
+
+    def __repr__(self):
+        return "Algorithm.%s()" % type(self).__name__
+
+
+class FFT(Alg):
+    """Filter using the Fast Fourier Transform."""
+
+
+class FIR(Alg):
+    """Filter using a direct algorithm."""
+
+
+class FIRTiled(Alg):
+    """Cache-efficient filtering using tiles (one fused GPU pass here)."""
+
+    def __init__(self, tilesize=()):
+        self.tilesize = tuple(tilesize)
+
+
+class IIR(Alg):
+    """Filter with an Infinite Impulse Response filter."""
+
+
+class Mixed(Alg):
+    """Filter with a cascade of mixed types (IIR, FIR)."""
+
+
+class Algorithm:
+    FFT, FIR, FIRTiled, IIR, Mixed = FFT, FIR, FIRTiled, IIR, Mixed
+
+
+class AbstractResource:
+    def __init__(self, settings=None):
+        self.settings = settings if settings is not None else FIR()
+
+    def __repr__(self):
+        return "%s(%r)" % (type(self).__name__, self.settings)
+
+
+class HIPDevice(AbstractResource):
+    """The MI355X (libifm).  This is where every call runs."""
+
+
+class CPU1(AbstractResource):
+    """Accepted for drop-in compatibility; the work runs on the GPU."""
+
+
+class CPUThreads(AbstractResource):
+    """Accepted for drop-in compatibility; the work runs on the GPU."""
+
+
+# ---- kernel canonicalisation ---------------------------------------------------------
+def _as_offset(k):
+    if isinstance(k, OffsetArray):
+        return k
+    a = np.asarray(k)
+    return OffsetArray(a, (1,) * a.ndim)   # a plain array inside a tuple keeps its 1-based axes
+
+
+def factorkernel(kernel):
+    """src/imfilter.jl:1156-1184: a 2-D kernel is tested for rank 1 with
+    svd (tolerance sqrt(eps)); rank 1 -> two factors scaled by sqrt(s)."""
+    if isinstance(kernel, Laplacian):
+        return (kernel,)
+    if isinstance(kernel, (ReshapedOneD, TriggsSdika)):
+        return (kernel,)
+    k = kernelshift(kernel) if not isinstance(kernel, OffsetArray) else kernel
+    if k.ndim != 2:
+        return (k,)
+    data = np.asarray(k.data)
+    m, n = data.shape
+    if data.dtype.kind in "iub":
+        data = data.astype(np.float64)
+    U, S, Vt = np.linalg.svd(data.astype(np.float64) if data.dtype != np.float32 else data)
+    eps = np.sqrt(np.finfo(S.dtype).eps)
+    separable = all(abs(S[i]) < eps for i in range(1, len(S)))
+    if not separable:
+        return (k,)   # the reference's leading dummykernel is a copy and is skipped
+    ss = np.sqrt(S[0])
+    u = (ss * U[:, 0:1]).astype(data.dtype)
+    v = (ss * Vt[0:1, :]).astype(data.dtype)
+    return (OffsetArray(u, (k.first[0], 0)), OffsetArray(v, (0, k.first[1])))
+
+
+def _kernel_dtype(k):
+    if isinstance(k, Laplacian):
+        return None
+    if isinstance(k, TriggsSdika):
+        return Float64
+    if isinstance(k, ReshapedOneD):
+        return Float64 if k.is_iir else eltype_of(np.asarray(k.data.data))
+    return eltype_of(np.asarray(k.data if isinstance(k, OffsetArray) else k))
+
+
+def filter_type(S, kernel):
+    """Output eltype (src/imfilter.jl:1131-1154)."""
+    ks = kernel if isinstance(kernel, tuple) else (kernel,)
+    T = None
+    for k in ks:
+        if isinstance(k, Laplacian):
+            t = S if S.kind == "float" else (Float32 if S is N0f8 else Int64)
+        else:
+            t = mul_type(S, _kernel_dtype(k))
+        T = t if T is None else promote(T, t)
+    return T
+
+
+class _Comp:
+    """One stage of a processed kernel on an N-dimensional image."""
+
+    def __init__(self, kind, axis=None, taps=None, first=None, dense=None, iir=None):
+        self.kind, self.axis, self.taps, self.first, self.dense, self.iir = kind, axis, taps, first, dense, iir
+
+
+def _components(kernel, N):
+    comps = []
+    for k in kernel:
+        if isinstance(k, Laplacian):
+            if len(k.flags) != N:
+                raise DimensionMismatch("Laplacian dimensionality must match the image")
+            comps.append(_Comp("dense", dense=_extend(k.to_array(), N)))
+            continue
+        if isinstance(k, TriggsSdika):
+            if N < 1:
+                raise DimensionMismatch("kernel dimensionality")
+            if not k.iscopy():
+                comps.append(_Comp("iir", axis=0, iir=k))
+            continue
+        if isinstance(k, ReshapedOneD):
+            if k.N > N:
+                raise DimensionMismatch("kernel has more dimensions than the image")
+            if k.is_iir:
+                if not k.data.iscopy():
+                    comps.append(_Comp("iir", axis=k.Npre, iir=k.data))
+                continue
+            taps = np.asarray(k.data.data, dtype=np.float64)
+            if len(taps) == 1 and k.data.first[0] == 0 and taps[0] == 1:
+                continue   # iscopy
+            comps.append(_Comp("fir", axis=k.Npre, taps=taps, first=k.data.first[0]))
+            continue
+        a = _extend(_as_offset(k), N)
+        ext = [d for d in range(N) if a.shape[d] > 1]
+        if all(s == 1 for s in a.shape) and all(f == 0 for f in a.first) and np.asarray(a.data).ravel()[0] == 1:
+            continue   # iscopy
+        shifts = [d for d in range(N) if a.shape[d] == 1 and a.first[d] != 0]
+        if len(ext) <= 1 and not shifts:
+            ax = ext[0] if ext else 0
+            comps.append(_Comp("fir", axis=ax, taps=np.asarray(a.data, dtype=np.float64).ravel(), first=a.first[ax]))
+        else:
+            comps.append(_Comp("dense", dense=a))
+    return comps
+
+
+def _extend(k, N):
+    if k.ndim > N:
+        raise DimensionMismatch("kernel has more dimensions (%d) than the image (%d)" % (k.ndim, N))
+    extra = N - k.ndim
+    return OffsetArray(np.asarray(k.data).reshape(k.shape + (1,) * extra), k.first + (0,) * extra)
+
+
+def _comp_dense(c, N):
+    if c.kind == "dense":
+        return OffsetArray(np.asarray(c.dense.data, dtype=np.float64), c.dense.first)
+    shape = [1] * N
+    shape[c.axis] = len(c.taps)
+    first = [0] * N
+    first[c.axis] = c.first
+    return OffsetArray(c.taps.reshape(shape), tuple(first))
+
+
+def kernelconv(comps, N):
+    """Full convolution of all FIR stages into one dense kernel
+    (src/imfilter.jl:1257-1270)."""
+    out = None
+    for c in comps:
+        k = _comp_dense(c, N)
+        if out is None:
+            out = k
+            continue
+        shape = tuple(a + b - 1 for a, b in zip(out.shape, k.shape))
+        acc = np.zeros(shape)
+        for idx in np.ndindex(*k.shape):
+            sl = tuple(slice(i, i + s) for i, s in zip(idx, out.shape))
+            acc[sl] += out.data * k.data[idx]
+        out = OffsetArray(acc, tuple(a + b for a, b in zip(out.first, k.first)))
+    return out
+
+
+def maxlen(kernel):
+    """src/imfilter.jl:1211-1216 (a dense array counts all its elements)."""
+    L = 0
+    for k in kernel:
+        if isinstance(k, ReshapedOneD):
+            L = max(L, len(k))
+        elif isinstance(k, (TriggsSdika, Laplacian)):
+            L = max(L, 1)
+        else:
+            L = max(L, int(np.asarray(k.data if isinstance(k, OffsetArray) else k).size))
+    return L
+
+
+def _all_finite(img):
+    if isinstance(img, DeviceArray):
+        return _device_all_finite(img)
+    et = eltype_of(img)
+    if et.kind != "float":
+        return True
+    a = np.asarray(img)
+    if et is BFloat16:
+        return bool(np.all((a & 0x7F80) != 0x7F80))
+    return bool(np.isfinite(a).all())
+
+
+def _device_all_finite(img):
+    if img.eltype.kind != "float":
+        return True
+    # the scan stays on the device: a min/max over the whole array propagates NaN/Inf
+    from .mapwindow import _finite_scan
+    return _finite_scan(img)
+
+
+def filter_algorithm(img, kernel):
+    """src/imfilter.jl:1200-1209."""
+    iir = [isinstance(k, TriggsSdika) or (isinstance(k, ReshapedOneD) and k.is_iir) for k in kernel]
+    if kernel and all(iir):
+        return IIR()
+    if any(iir):
+        return Mixed()
+    if maxlen(kernel) > 30 and _all_finite(img):
+        return FFT()
+    return FIRTiled() if len(kernel) > 1 else FIR()
+
+
+# ---- the GPU calls -----------------------------------------------------------------
+def _border_struct(border, S):
+    b = _lib.IfmBorder()
+    if isinstance(border, Fill):
+        b.style = _lib.FILL
+        b.fill = convert_fill(border.value, S)
+    elif isinstance(border, Pad):
+        if border.style not in STYLES:
+            raise ValueError("border style %s unrecognized" % border.style)
+        b.style = STYLES[border.style]
+    else:
+        b.style = _lib.REPLICATE
+    return b
+
+
+def _required(kernel, N):
+    inds = calculate_padding(kernel)
+    inds = tuple(inds) + ((0, 0),) * (N - len(inds))
+    return inds
+
+
+def _check_explicit_padding(border, kernel, N):
+    if border.lo is None:
+        return
+    if len(border.lo) != N:
+        raise ArgumentError("%r lacks the proper padding sizes for an array with %d dimensions" % (border, N))
+    for d, (f, l) in enumerate(_required(kernel, N)):
+        if border.lo[d] < max(-f, 0) or border.hi[d] < max(l, 0):
+            raise DimensionMismatch("requested indices and kernel indices do not agree with indices of padded input")
+
+
+def _run(alg, out, img, kernel, border, out_first):
+    N = len(img.shape)
+    S = eltype_of(img)
+    comps = _components(kernel, N)
+    b = _border_struct(border, S)
+    of = _lib.i64(out_first) if out_first is not None else None
+    ai, ao = ifm_array(img), ifm_array(out)
+    L = _lib.load()
+    ctx = _lib.ctx()
+    if not comps:
+        fa = (_lib.IfmFactor * 1)()
+        _lib.check(L.ifm_imfilter_fir(ctx, ai, ao, of, fa, 0, b))
+        return out
+    kinds = {c.kind for c in comps}
+    if isinstance(alg, IIR) or kinds == {"iir"}:
+        if kinds != {"iir"}:
+            raise NotImplementedError("IIR algorithm with FIR factors (Algorithm.Mixed) is not implemented")
+        if isinstance(border, Pad) and border.lo is not None and any(border.lo + border.hi):
+            pass
+        arr = (_lib.IfmIir * len(comps))()
+        for i, c in enumerate(comps):
+            k = c.iir
+            for j in range(3):
+                arr[i].a[j], arr[i].b[j] = float(k.a[j]), float(k.b[j])
+            arr[i].scale = k.scale
+            for j in range(9):
+                arr[i].M[j] = float(np.asarray(k.M).ravel()[j])
+            arr[i].axis = c.axis
+        _lib.check(L.ifm_imfilter_iir(ctx, ai, ao, arr, len(comps), b))
+        return out
+    if "iir" in kinds:
+        raise NotImplementedError("mixed IIR/FIR cascades (Algorithm.Mixed) are not implemented")
+    if isinstance(alg, FFT):
+        k = kernelconv(comps, N)
+        kk = _kernel_struct(k)
+        _lib.check(L.ifm_imfilter_fft(ctx, ai, ao, of, kk, b))
+        return out
+    if kinds == {"fir"}:
+        fa = (_lib.IfmFactor * len(comps))()
+        keep = []
+        for i, c in enumerate(comps):
+            t = np.ascontiguousarray(c.taps, dtype=np.float64)
+            keep.append(t)
+            fa[i].axis, fa[i].len, fa[i].first = c.axis, len(t), c.first
+            fa[i].taps = t.ctypes.data_as(_lib.C.POINTER(_lib.C.c_double))
+        rc = L.ifm_imfilter_fir(ctx, ai, ao, of, fa, len(comps), b)
+        if rc == _lib.E_UNSUPPORTED:       # a factor longer than the FIR kernels take
+            k = kernelconv(comps, N)
+            _lib.check(L.ifm_imfilter_fft(ctx, ai, ao, of, _kernel_struct(k), b))
+            return out
+        _lib.check(rc)
+        return out
+    k = kernelconv(comps, N)
+    _lib.check(L.ifm_imfilter_dense(ctx, ai, ao, of, _kernel_struct(k), b, _lib.PREC_AUTO))
+    return out
+
+
+def _kernel_struct(k):
+    kk = _lib.IfmKernel()
+    data = np.asfortranarray(np.asarray(k.data, dtype=np.float64))
+    kk.ndim = data.ndim
+    for d in range(data.ndim):
+        kk.size[d] = data.shape[d]
+        kk.first[d] = k.first[d]
+    flat = np.ascontiguousarray(data.ravel(order="F"))
+    kk.taps = flat.ctypes.data_as(_lib.C.POINTER(_lib.C.c_double))
+    kk._keep = flat   # the taps live as long as the struct
+    return kk
+
+
+# ---- public API ------------------------------------------------------------------------
+def _parse_imfilter(args):
+    args = list(args)
+    r = args.pop(0) if args and isinstance(args[0], AbstractResource) else None
+    T = as_eltype(args.pop(0)) if args and is_eltype_spec(args[0]) else None
+    if len(args) < 2:
+        raise MethodError("imfilter([r], [T], img, kernel, [border], [alg])")
+    img, kernel, rest = args[0], args[1], args[2:]
+    border = rest[0] if len(rest) >= 1 else "replicate"
+    alg = rest[1] if len(rest) >= 2 else None
+    if len(rest) > 2 or (r is not None and alg is not None):
+        raise MethodError("no method matching this imfilter call")
+    if alg is not None and not isinstance(alg, Alg):
+        raise MethodError("the algorithm must be an Algorithm.* instance")
+    return r, T, img, kernel, border, alg
+
+
+def _processed(kernel):
+    if isinstance(kernel, tuple):
+        return tuple(_as_offset(k) if isinstance(k, np.ndarray) else k for k in kernel)
+    return factorkernel(kernel)
+
+
+def imfilter(*args):
+    """imfilter([r], [T], img, kernel, [border="replicate"], [alg]) (src/imfilter.jl:1-49)."""
+    r, T, img, kernel, border, alg = _parse_imfilter(args)
+    if isinstance(kernel, TriggsSdika):
+        kernel = (kernel,)
+    kernel = _processed(kernel)
+    img = img if isinstance(img, DeviceArray) else host_dense(img)
+    S = eltype_of(img)
+    if T is None:
+        T = filter_type(S, kernel)
+    if T.code is None or T in (UInt8, N0f8):
+        raise NotImplementedError("integer output eltype %s: pass a floating-point T" % T)
+    border = borderinstance(border)
+    if isinstance(border, NA):
+        raise NotImplementedError("NA() borders are not implemented")
+    N = len(img.shape)
+    if isinstance(border, Inner):
+        lo, hi = _inner_padding(border, kernel, N)
+        shape = tuple(n - l - h for n, l, h in zip(img.shape, lo, hi))
+        if any(s < 0 for s in shape):
+            raise DimensionMismatch("kernel larger than the image for Inner()")
+        out = empty_like(img, T, shape)
+        _run(_choose(r, alg, img, kernel), out, img, kernel, Pad("replicate"), lo)
+        return OffsetArray(out, tuple(lo))
+    if isinstance(border, (Pad, Fill)):
+        _check_explicit_padding(border, kernel, N)
+    out = empty_like(img, T)
+    return _run(_choose(r, alg, img, kernel), out, img, kernel, border, None)
+
+
+def _inner_padding(border, kernel, N):
+    if border.lo is None:
+        inds = _required(kernel, N)
+        return [max(-f, 0) for f, _ in inds], [max(l, 0) for _, l in inds]
+    if len(border.lo) != N:
+        raise DimensionMismatch("dimensionality of img and the border must agree, got %d and %d" % (N, len(border.lo)))
+    for d, (f, l) in enumerate(_required(kernel, N)):
+        if border.lo[d] < max(-f, 0) or border.hi[d] < max(l, 0):
+            raise DimensionMismatch("requested indices and kernel indices do not agree with indices of padded input")
+    return list(border.lo), list(border.hi)
+
+
+def _choose(r, alg, img, kernel):
+    if alg is None and r is not None:
+        alg = r.settings
+    if alg is None:
+        alg = filter_algorithm(img, kernel)
+    return alg
+
+
+def imfilter_(*args):
+    """imfilter!(out, img, kernel, [border], [alg]), imfilter!(r, out, img, kernel, [border]),
+    imfilter!(r, out, img, kernel::TriggsSdika, dim, border) and
+    imfilter!(r, out, A, kernel, NoPad(), [inds]) (src/imfilter.jl:204-254, :922-932)."""
+    args = list(args)
+    r = args.pop(0) if args and isinstance(args[0], AbstractResource) else None
+    if len(args) < 3:
+        raise MethodError("imfilter!([r], out, img, kernel, ...)")
+    out, img, kernel, rest = args[0], args[1], args[2], args[3:]
+    if isinstance(kernel, TriggsSdika) and rest and isinstance(rest[0], (int, np.integer)):
+        dim = int(rest[0])                                  # 1-based like the reference
+        border = rest[1] if len(rest) > 1 else "replicate"
+        N = len(img.shape)
+        comp = ReshapedOneD(N, dim - 1, kernel)
+        return _run(IIR(), out, host_dense(img) if not isinstance(img, DeviceArray) else img, (comp,),
+                    borderinstance(border) if isinstance(border, str) else border, None)
+    border = rest[0] if len(rest) >= 1 else "replicate"
+    alg = rest[1] if len(rest) >= 2 else None
+    if r is not None and len(rest) >= 2 and not isinstance(border, NoPad):
+        raise MethodError("no method matching imfilter!(r, out, img, kernel, border, alg)")
+    if isinstance(kernel, TriggsSdika):
+        kernel = (kernel,)
+    kernel = _processed(kernel)
+    img = img if isinstance(img, DeviceArray) else host_dense(img)
+    N = len(img.shape)
+    if isinstance(border, NoPad):
+        return _nopad(out, img, kernel, rest[1] if len(rest) >= 2 else None, r)
+    border = borderinstance(border)
+    if isinstance(border, NA):
+        raise NotImplementedError("NA() borders are not implemented")
+    if isinstance(border, Inner):
+        lo, hi = _inner_padding(border, kernel, N)
+        _run(_choose(r, alg, img, kernel), _outdata(out), img, kernel, Pad("replicate"), lo)
+        return out
+    if isinstance(border, (Pad, Fill)):
+        _check_explicit_padding(border, kernel, N)
+    if tuple(np.shape(_outdata(out))) != tuple(img.shape):
+        raise DimensionMismatch("output axes %r disagree with the image axes %r" % (np.shape(out), img.shape))
+    _run(_choose(r, alg, img, kernel), _outdata(out), img, kernel, border, None)
+    return out
+
+
+def _outdata(out):
+    return out.data if isinstance(out, OffsetArray) else out
+
+
+def _nopad(out, A, kernel, inds, r):
+    """imfilter!(r, out, A, kernel, NoPad(), inds): A is already padded; every
+    output index must be computable in bounds (src/imfilter.jl:592-617)."""
+    a_first = A.first if isinstance(A, OffsetArray) else (0,) * len(np.shape(A))
+    A_data = A.data if isinstance(A, OffsetArray) else A
+    o_first = out.first if isinstance(out, OffsetArray) else (0,) * len(np.shape(out))
+    o_data = _outdata(out)
+    N = len(np.shape(A_data))
+    if inds is None:
+        inds = tuple(range(f, f + n) for f, n in zip(o_first, np.shape(o_data)))
+    for d in range(N):
+        if inds[d].start < o_first[d] or inds[d].stop > o_first[d] + np.shape(o_data)[d]:
+            raise DimensionMismatch("output indices disagree with requested indices")
+    req = _required(kernel, N)
+    for d, (f, l) in enumerate(req):
+        if a_first[d] > inds[d].start + f or a_first[d] + np.shape(A_data)[d] - 1 < inds[d].stop - 1 + l:
+            raise DimensionMismatch("requested indices and kernel indices do not agree with indices of padded input")
+    full = all(inds[d].start == o_first[d] and len(inds[d]) == np.shape(o_data)[d] for d in range(N))
+    if not full:
+        raise NotImplementedError("NoPad with a sub-range of the output")
+    of = [inds[d].start - a_first[d] for d in range(N)]
+    alg = r.settings if r is not None else FIR()
+    _run(alg, o_data, host_dense(A_data) if not isinstance(A_data, DeviceArray) else A_data, kernel,
+         Pad("replicate"), of)
+    return out

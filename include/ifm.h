@@ -1,0 +1,201 @@
+/*
+ * ifm.h — C ABI of libifm, the MI355X (gfx950) filtering library that backs the
+ * ImageFiltering.jl hot path: imfilter / imfilter! (FIR, FFT, IIR), padarray and
+ * the mapwindow median / min / max / extrema filters.
+ *
+ * Conventions
+ *   - Axis d of an ifm_array is Julia dimension d+1.  Sizes and strides are in
+ *     elements.  Any dense layout is accepted (column-major, row-major or any
+ *     other axis permutation); input and output must share the permutation.
+ *   - All indices are 0-based.  A Julia axis first(inds) is passed as the
+ *     0-based offset of the output region (out_first) or of a kernel's first
+ *     tap (ifm_factor.first, ifm_kernel.first).
+ *   - data points to device memory when device == 1 and to host memory when
+ *     device == 0 (the library then stages the array through HBM and the call
+ *     returns after the result has been copied back).
+ *   - Calls on one ctx are serialised on the ctx's stream.  Calls without a
+ *     server return once the work is enqueued on the stream (device arrays) or
+ *     finished (host arrays); ifm_ctx_synchronize waits for the stream.
+ *   - Every entry point returns an ifm_status; ifm_last_error gives the message.
+ *     IFM_E_ARG maps to Julia's ArgumentError and IFM_E_DIM to DimensionMismatch.
+ *   - Border Fill values arrive already converted to the input eltype
+ *     (src/borderarray.jl:11-24); they are passed as a double, which holds
+ *     every value of every supported eltype exactly.
+ */
+#ifndef IFM_H
+#define IFM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IFM_MAXDIM 4
+
+typedef enum {
+  IFM_OK = 0,
+  IFM_E_ARG = 1,          /* ArgumentError */
+  IFM_E_DIM = 2,          /* DimensionMismatch */
+  IFM_E_NOMEM = 3,        /* device allocation failed */
+  IFM_E_DEVICE = 4,       /* HIP runtime error */
+  IFM_E_UNSUPPORTED = 5,  /* eltype / rank / layout not supported by a kernel */
+  IFM_E_BUDGET = 6        /* serving: request exceeds the HBM budget */
+} ifm_status;
+
+/* Element types.  N0F8 is UInt8 storage read as code/255 by linear filters;
+ * the order filters (median, min, max, extrema) work on the raw codes. */
+typedef enum {
+  IFM_U8 = 0, IFM_N0F8 = 1, IFM_F32 = 2, IFM_F64 = 3, IFM_BF16 = 4, IFM_F16 = 5
+} ifm_dtype;
+
+/* Border styles (src/border.jl:564-590, src/borderarray.jl:110-149). */
+typedef enum {
+  IFM_REPLICATE = 0, IFM_CIRCULAR = 1, IFM_SYMMETRIC = 2, IFM_REFLECT = 3, IFM_FILL = 4
+} ifm_style;
+
+/* Dense-kernel arithmetic for ifm_imfilter_dense. */
+typedef enum {
+  IFM_PREC_AUTO = 0,    /* library choice: MFMA split-bf16 for kernels >= 15x15, else FP32 VALU */
+  IFM_PREC_FP32 = 1,    /* FP32 FMA on the vector ALUs */
+  IFM_PREC_BF16X3 = 2   /* MFMA, x and k split in bf16 hi+lo, 3 products, FP32 accumulate */
+} ifm_precision;
+
+typedef struct {
+  int32_t ndim;                 /* 1..IFM_MAXDIM */
+  int32_t dtype;                /* ifm_dtype */
+  int64_t size[IFM_MAXDIM];
+  int64_t stride[IFM_MAXDIM];
+  void *data;
+  int32_t device;               /* 1: device pointer, 0: host pointer */
+  int32_t reserved;
+} ifm_array;
+
+typedef struct {
+  int32_t style;                /* ifm_style */
+  int32_t reserved;
+  double fill;                  /* value for IFM_FILL */
+  int64_t lo[IFM_MAXDIM];       /* explicit padding, used by ifm_padarray only */
+  int64_t hi[IFM_MAXDIM];
+} ifm_border;
+
+/* One 1-D factor of a separable kernel: a ReshapedOneD along `axis`
+ * (src/kernelfactors.jl:52-131) with taps k[first .. first+len-1]. */
+typedef struct {
+  int32_t axis;
+  int32_t len;
+  int64_t first;
+  const double *taps;           /* host memory, len values */
+} ifm_factor;
+
+/* A dense N-D kernel with axes first[d] .. first[d]+size[d]-1, taps stored
+ * column-major (axis 0 fastest) in host memory. */
+typedef struct {
+  int32_t ndim;
+  int32_t reserved;
+  int64_t size[IFM_MAXDIM];
+  int64_t first[IFM_MAXDIM];
+  const double *taps;
+} ifm_kernel;
+
+/* A Triggs-Sdika IIR factor (src/kernelfactors.jl:463-513): forward
+ * coefficients a, backward b, final scale and the 3x3 boundary matrix M
+ * (row-major, M[3*r + c]) applied along `axis`. */
+typedef struct {
+  double a[3];
+  double b[3];
+  double scale;
+  double M[9];
+  int32_t axis;
+  int32_t reserved;
+} ifm_iir;
+
+typedef struct ifm_ctx ifm_ctx;
+
+/* ---- context ------------------------------------------------------------ */
+/* stream: a hipStream_t (may be NULL for the null stream). */
+int ifm_ctx_create(int device, void *stream, ifm_ctx **out);
+int ifm_ctx_destroy(ifm_ctx *ctx);
+int ifm_ctx_set_stream(ifm_ctx *ctx, void *stream);
+int ifm_ctx_synchronize(ifm_ctx *ctx);
+const char *ifm_last_error(const ifm_ctx *ctx);
+/* Library version string, e.g. "ifm 0.1 gfx950". */
+const char *ifm_version(void);
+
+/* ---- padarray ------------------------------------------------------------ */
+/* Replaces padarray(T, img, border) -> BorderArray + copy!
+ * (src/border.jl:324-347, src/borderarray.jl:157-196).  out.size[d] must be
+ * lo[d] + in.size[d] + hi[d]; in and out may differ in eltype (convert). */
+int ifm_padarray(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const ifm_border *border);
+
+/* ---- imfilter: FIR ------------------------------------------------------- */
+/* Replaces the cascade of 1-D factors imfilter!(r, out, padarray(img),
+ * (f1, f2, ...), NoPad(border)) (src/imfilter.jl:331-457, :671-739).
+ * out covers image indices out_first[d] .. out_first[d]+out.size[d]-1
+ * (NULL out_first = 0, i.e. the full image as in imfilter; Inner() passes its
+ * interior).  Factors may repeat an axis.  The border map is applied on load,
+ * so no padded copy is made. */
+int ifm_imfilter_fir(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const int64_t *out_first,
+                     const ifm_factor *factors, int nfactors, const ifm_border *border);
+
+/* Replaces the dense N-D correlation __imfilter_inbounds!
+ * (src/imfilter.jl:592-669) on the padded image. */
+int ifm_imfilter_dense(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const int64_t *out_first,
+                       const ifm_kernel *kernel, const ifm_border *border, int precision);
+
+/* ---- imfilter: FFT ------------------------------------------------------- */
+/* Replaces _imfilter_fft! / filtfft (src/imfilter.jl:796-866) with padding
+ * to nextprod([2,3], n + K) per axis as Pad(kernel, img, FFT())
+ * (src/border.jl:240-248).  Up to 3 dims. */
+int ifm_imfilter_fft(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const int64_t *out_first,
+                     const ifm_kernel *kernel, const ifm_border *border);
+
+/* ---- imfilter: IIR ------------------------------------------------------- */
+/* Replaces _imfilter_dim! with leftborder!/rightborder!
+ * (src/imfilter.jl:964-1092) applied once per factor, in order.  The
+ * recursion runs in FP64.  A Pad-style border initialises from the edge
+ * pixel, IFM_FILL from border->fill.  in and out may alias. */
+int ifm_imfilter_iir(ifm_ctx *ctx, const ifm_array *in, ifm_array *out,
+                     const ifm_iir *factors, int nfactors, const ifm_border *border);
+
+/* ---- mapwindow ------------------------------------------------------------ */
+/* Replaces mapwindow(median!, img, window; border) -> median_fast!
+ * (src/mapwindow.jl:123-131, :270-333): the ((L+1)÷2)-th smallest of the
+ * L window values under isless order.  win_first[d]/win_len[d] give the window
+ * offsets.  out holds the selected element in the input eltype. */
+int ifm_mapwindow_median(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const int64_t *out_first,
+                         const int64_t *win_first, const int64_t *win_len, const ifm_border *border);
+
+/* Replaces mapwindow(minimum|maximum, img, window; border) and
+ * mapwindow(extrema, img, window; border) (src/mapwindow.jl:270-333).
+ * Either of omin / omax may be NULL. */
+int ifm_mapwindow_minmax(ifm_ctx *ctx, const ifm_array *in, ifm_array *omin, ifm_array *omax,
+                         const int64_t *out_first, const int64_t *win_first, const int64_t *win_len,
+                         const ifm_border *border);
+
+/* Replaces mapwindow(extrema, img, window::Dims) -> extrema_filter
+ * (src/mapwindow.jl:337-481): truncated windows of width window[d] (even
+ * widths allowed), independent of any border. */
+int ifm_extrema_filter(ifm_ctx *ctx, const ifm_array *in, ifm_array *omin, ifm_array *omax,
+                       const int64_t *window);
+
+/* ---- device memory plumbing ------------------------------------------------ */
+/* Helpers for hosts without their own HIP allocator (the Python package and
+ * the Julia binding use them for device-resident arrays).  Copies are
+ * synchronous with respect to the ctx stream. */
+int ifm_device_alloc(ifm_ctx *ctx, uint64_t bytes, void **ptr);
+int ifm_device_free(ifm_ctx *ctx, void *ptr);
+int ifm_memcpy_h2d(ifm_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+int ifm_memcpy_d2h(ifm_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+int ifm_memset_d(ifm_ctx *ctx, void *dst, int value, uint64_t bytes);
+/* HIP events on the ctx stream: ms between the last start and stop. */
+int ifm_timer_start(ifm_ctx *ctx);
+int ifm_timer_stop(ifm_ctx *ctx, float *ms);
+/* Number of visible devices (0 when no GPU); never initialises a context. */
+int ifm_device_count(int *count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* IFM_H */

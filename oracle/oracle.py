@@ -1,0 +1,190 @@
+"""ctypes front end of the CPU oracle (oracle/ifm_oracle.c).
+
+Test infrastructure only: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product.  Arrays follow the same
+convention as the product's Python host layer: numpy axis d is Julia
+dimension d+1, 0-based indices, results in FP64 (or the input codes for the
+order filters).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+STYLES = {"replicate": 0, "circular": 1, "symmetric": 2, "reflect": 3, "fill": 4}
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
+        L = C.CDLL(path)
+        i64p = C.POINTER(C.c_int64)
+        dp = C.POINTER(C.c_double)
+        ip = C.POINTER(C.c_int)
+        L.orc_padmap.restype = C.c_int64
+        L.orc_padmap.argtypes = [C.c_int, C.c_int64, C.c_int64]
+        L.orc_padindex.argtypes = [C.c_int, C.c_int64, C.c_int64, C.c_int64, i64p]
+        L.orc_padarray_f64.argtypes = [dp, C.c_int, i64p, C.c_int, C.c_double, i64p, i64p, dp]
+        L.orc_imfilter_dense.argtypes = [dp, C.c_int, i64p, dp, i64p, i64p, C.c_int, C.c_double,
+                                         i64p, i64p, dp]
+        L.orc_imfilter_factors.argtypes = [dp, C.c_int, i64p, C.c_int, ip, ip, i64p, dp, C.c_int,
+                                           C.c_double, i64p, i64p, dp]
+        L.orc_imfilter_iir.argtypes = [dp, C.c_int, i64p, C.c_int, dp, ip, C.c_int, C.c_double, dp]
+        L.orc_median_f64.argtypes = [dp, C.c_int, i64p, i64p, i64p, C.c_int, C.c_double, i64p, i64p, dp]
+        L.orc_minmax_f64.argtypes = [dp, C.c_int, i64p, i64p, i64p, C.c_int, C.c_double, i64p, i64p,
+                                     dp, dp]
+        L.orc_extrema_lemire_f64.argtypes = [dp, C.c_int, i64p, i64p, dp, dp]
+        L.orc_imfilter_dense_rows.argtypes = [dp, C.c_int, i64p, dp, i64p, i64p, C.c_int, C.c_double,
+                                              C.c_int64, C.c_int64, dp]
+        _LIB = L
+    return _LIB
+
+
+def _i64(v):
+    a = (C.c_int64 * max(1, len(v)))(*[int(x) for x in v])
+    return a
+
+
+def _f64(a):
+    a = np.asfortranarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _outbuf(shape):
+    o = np.zeros(shape, dtype=np.float64, order="F")
+    return o, o.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _style(border):
+    if isinstance(border, str):
+        return STYLES[border], 0.0
+    kind, val = border  # ("fill", value)
+    return 4, float(val)
+
+
+def padindex(style, lo, n, hi):
+    out = (C.c_int64 * (lo + n + hi))()
+    if lib().orc_padindex(STYLES[style], lo, n, hi, out) != 0:
+        raise ZeroDivisionError("reflect padding of a length-1 axis")
+    return np.array(list(out), dtype=np.int64)
+
+
+def padarray(img, border, lo, hi):
+    a, ap = _f64(img)
+    s, fill = _style(border)
+    shape = tuple(lo[d] + a.shape[d] + hi[d] for d in range(a.ndim))
+    o, op = _outbuf(shape)
+    if lib().orc_padarray_f64(ap, a.ndim, _i64(a.shape), s, fill, _i64(lo), _i64(hi), op) != 0:
+        raise ZeroDivisionError("reflect padding of a length-1 axis")
+    return o
+
+
+def _region(a, ofirst, osize):
+    of = ofirst if ofirst is not None else [0] * a.ndim
+    os_ = osize if osize is not None else list(a.shape)
+    return of, os_
+
+
+def imfilter_dense(img, k, kfirst, border, ofirst=None, osize=None):
+    """Dense correlation of img with kernel k whose axis d starts at kfirst[d]."""
+    a, ap = _f64(img)
+    kk, kp = _f64(k)
+    s, fill = _style(border)
+    of, os_ = _region(a, ofirst, osize)
+    o, op = _outbuf(tuple(os_))
+    if lib().orc_imfilter_dense(ap, a.ndim, _i64(a.shape), kp, _i64(kk.shape), _i64(kfirst), s, fill,
+                                _i64(of), _i64(os_), op) != 0:
+        raise ZeroDivisionError("reflect padding of a length-1 axis")
+    return o
+
+
+def imfilter_factors(img, factors, border, ofirst=None, osize=None):
+    """Cascade of 1-D factors: list of (axis, taps, first)."""
+    a, ap = _f64(img)
+    s, fill = _style(border)
+    nf = len(factors)
+    faxis = (C.c_int * max(1, nf))(*[f[0] for f in factors])
+    flen = (C.c_int * max(1, nf))(*[len(f[1]) for f in factors])
+    ffirst = _i64([f[2] for f in factors])
+    taps = np.ascontiguousarray(np.concatenate([np.asarray(f[1], dtype=np.float64) for f in factors])
+                                if nf else np.zeros(1))
+    of, os_ = _region(a, ofirst, osize)
+    o, op = _outbuf(tuple(os_))
+    rc = lib().orc_imfilter_factors(ap, a.ndim, _i64(a.shape), nf, faxis, flen, ffirst,
+                                    taps.ctypes.data_as(C.POINTER(C.c_double)), s, fill,
+                                    _i64(of), _i64(os_), op)
+    if rc != 0:
+        raise ZeroDivisionError("reflect padding of a length-1 axis")
+    return o
+
+
+def imfilter_iir(img, kernels, border):
+    """kernels: list of (axis, a[3], b[3], scale, M 3x3)."""
+    a, ap = _f64(img)
+    nk = len(kernels)
+    coef = np.zeros((max(1, nk), 16))
+    for i, (ax, ka, kb, sc, M) in enumerate(kernels):
+        coef[i, 0:3] = ka
+        coef[i, 3:6] = kb
+        coef[i, 6] = sc
+        coef[i, 7:16] = np.asarray(M, dtype=np.float64).reshape(9)
+    coef = np.ascontiguousarray(coef)
+    axes = (C.c_int * max(1, nk))(*[k[0] for k in kernels])
+    isfill = 0
+    fill = 0.0
+    if not isinstance(border, str):
+        isfill, fill = 1, float(border[1])
+    o, op = _outbuf(a.shape)
+    rc = lib().orc_imfilter_iir(ap, a.ndim, _i64(a.shape), nk, coef.ctypes.data_as(C.POINTER(C.c_double)),
+                                axes, isfill, fill, op)
+    if rc != 0:
+        raise ValueError("IIR axis too short")
+    return o
+
+
+def median(img, wfirst, wlen, border, ofirst=None, osize=None):
+    a, ap = _f64(img)
+    s, fill = _style(border)
+    of, os_ = _region(a, ofirst, osize)
+    o, op = _outbuf(tuple(os_))
+    lib().orc_median_f64(ap, a.ndim, _i64(a.shape), _i64(wfirst), _i64(wlen), s, fill, _i64(of),
+                         _i64(os_), op)
+    return o
+
+
+def minmax(img, wfirst, wlen, border, ofirst=None, osize=None):
+    a, ap = _f64(img)
+    s, fill = _style(border)
+    of, os_ = _region(a, ofirst, osize)
+    omin, pmin = _outbuf(tuple(os_))
+    omax, pmax = _outbuf(tuple(os_))
+    lib().orc_minmax_f64(ap, a.ndim, _i64(a.shape), _i64(wfirst), _i64(wlen), s, fill, _i64(of),
+                         _i64(os_), pmin, pmax)
+    return omin, omax
+
+
+def extrema_lemire(img, window):
+    a, ap = _f64(img)
+    omin, pmin = _outbuf(a.shape)
+    omax, pmax = _outbuf(a.shape)
+    lib().orc_extrema_lemire_f64(ap, a.ndim, _i64(a.shape), _i64(window), pmin, pmax)
+    return omin, omax
+
+
+def imfilter_dense_rows(img, k, kfirst, border, row0, nrows):
+    """Dense FIR over rows row0..row0+nrows-1 of axis 1 (CPU-baseline timing)."""
+    a, ap = _f64(img)
+    kk, kp = _f64(k)
+    s, fill = _style(border)
+    shape = list(a.shape)
+    shape[1] = nrows
+    o, op = _outbuf(tuple(shape))
+    lib().orc_imfilter_dense_rows(ap, a.ndim, _i64(a.shape), kp, _i64(kk.shape), _i64(kfirst), s, fill,
+                                  row0, nrows, op)
+    return o

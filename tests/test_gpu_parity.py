@@ -1,0 +1,399 @@
+"""GPU parity: the HIP path (through the public API and the C ABI) against the
+CPU oracle on the same seeded inputs.  Integer / index / order-statistic work
+must be bit-exact; floating-point filtering must stay within a normwise
+relative error of 1e-5 of the FP64 oracle: max|y - y64| / max|y64| <= 1e-5."""
+import numpy as np
+import pytest
+
+import imagefiltering as IF
+from imagefiltering import Kernel, KernelFactors
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+STYLES = ["replicate", "circular", "symmetric", "reflect"]
+
+
+def relerr(y, y64):
+    y = np.asarray(y, dtype=np.float64)
+    y64 = np.asarray(y64, dtype=np.float64)
+    scale = np.max(np.abs(y64))
+    return float(np.max(np.abs(y - y64)) / (scale if scale > 0 else 1.0))
+
+
+def obord(b):
+    if isinstance(b, IF.Fill):
+        return ("fill", float(b.value))
+    return b
+
+
+def factors_of(kern):
+    out = []
+    for f in kern:
+        out.append((f.Npre, f.data.data.astype(np.float64), f.data.first[0]))
+    return out
+
+
+# ---- padarray ----------------------------------------------------------------------------------
+@pytest.mark.parametrize("style", STYLES + ["fill"])
+def test_padarray_reference_matrices(style):
+    import json, os
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_fixtures.json")))
+    A = np.arange(1, 26, dtype=np.float64).reshape(5, 5, order="F")
+    b = IF.Fill(0, (2, 2), (2, 2)) if style == "fill" else IF.Pad(style, (2, 2), (2, 2))
+    P = IF.padarray(A, b)
+    assert P.first == (-2, -2)
+    np.testing.assert_array_equal(P.data, np.array(g["padarray_5x5"]["fill0" if style == "fill" else style]))
+    A2 = np.array([[1.0, 2.0], [3.0, 4.0]])
+    if style != "fill":
+        np.testing.assert_array_equal(IF.padarray(A2, IF.Pad(style, (3, 3), (3, 3))).data,
+                                      np.array(g["padarray_2x2_pad3"][style]))
+
+
+@pytest.mark.parametrize("dtype", ["f32", "f64", "u8", "n0f8", "bf16"])
+def test_padarray_random_all_styles_bit_exact(dtype):
+    rng = np.random.default_rng(11)
+    for trial in range(12):
+        nd = 1 + trial % 3
+        shape = tuple(int(rng.integers(2, 9)) for _ in range(nd))
+        lo = tuple(int(rng.integers(0, 12)) for _ in range(nd))
+        hi = tuple(int(rng.integers(0, 12)) for _ in range(nd))
+        x = rng.random(shape)
+        if dtype == "f32":
+            a, ref = x.astype(np.float32), x.astype(np.float32)
+        elif dtype == "f64":
+            a, ref = x, x
+        elif dtype == "u8":
+            a = (x * 255).astype(np.uint8)
+            ref = a
+        elif dtype == "n0f8":
+            a = IF.n0f8((x * 255).astype(np.uint8))
+            ref = np.asarray(a)
+        else:
+            a = IF.bf16(x)
+            ref = np.asarray(a)
+        for style in STYLES + ["fill"]:
+            b = IF.Fill(0, lo, hi) if style == "fill" else IF.Pad(style, lo, hi)
+            P = IF.padarray(a, b)
+            want = O.padarray(ref.astype(np.float64), obord(b) if style == "fill" else style, list(lo), list(hi))
+            np.testing.assert_array_equal(np.asarray(P.data).astype(np.float64), want, err_msg="%s %s" % (style, shape))
+
+
+def test_padarray_conversion_and_layout():
+    x = np.asfortranarray(np.random.default_rng(1).random((7, 5)).astype(np.float32))
+    P = IF.padarray(IF.Float64, x, IF.Pad("symmetric", (3, 1), (2, 4)))
+    want = O.padarray(x.astype(np.float64), "symmetric", [3, 1], [2, 4])
+    np.testing.assert_array_equal(P.data, want)
+    assert P.data.dtype == np.float64
+    xc = np.ascontiguousarray(x)
+    np.testing.assert_array_equal(IF.padarray(xc, IF.Pad("reflect", (3, 1), (2, 4))).data,
+                                  O.padarray(xc.astype(np.float64), "reflect", [3, 1], [2, 4]))
+
+
+# ---- separable FIR ---------------------------------------------------------------------------
+def test_config1_gaussian_1024_replicate_f64_out():
+    """imfilter(img, KernelFactors.gaussian((3.0,3.0)), "replicate") on a 1024x1024
+    Float32 image: Float64 output within 1e-5 of the FP64 oracle."""
+    rng = np.random.default_rng(20261203)
+    img = rng.random((1024, 1024), dtype=np.float32)
+    kern = KernelFactors.gaussian((3.0, 3.0))
+    out = IF.imfilter(img, kern, "replicate")
+    assert out.dtype == np.float64 and out.shape == img.shape
+    want = O.imfilter_factors(img.astype(np.float64), factors_of(kern), "replicate")
+    assert relerr(out, want) <= TOL
+
+
+@pytest.mark.parametrize("border", STYLES + [IF.Fill(0.3)])
+def test_separable_all_borders_random_shapes(border):
+    rng = np.random.default_rng(12)
+    for shape in [(37, 29), (5, 3), (64, 17), (130, 70)]:
+        img = rng.random(shape).astype(np.float32)
+        for sig in [(1.0, 2.0), (0.7, 3.3)]:
+            kern = KernelFactors.gaussian(sig)
+            out = IF.imfilter(img, kern, border)
+            want = O.imfilter_factors(img.astype(np.float64), factors_of(kern), obord(border))
+            assert relerr(out, want) <= TOL, (shape, sig, border)
+
+
+def test_separable_offsets_repeats_and_inner():
+    rng = np.random.default_rng(13)
+    img = rng.random((40, 33))
+    k1 = IF.OffsetArray(np.array([0.2, 0.8]), (-1,))
+    k2 = IF.OffsetArray(np.array([0.3, 0.6, 0.1]), (1,))
+    kern = (IF.KernelFactors.ReshapedOneD(2, 0, k1), IF.KernelFactors.ReshapedOneD(2, 1, k2),
+            IF.KernelFactors.ReshapedOneD(2, 0, k1))
+    for b in STYLES + [IF.Fill(-1.0)]:
+        out = IF.imfilter(img, kern, b)
+        want = O.imfilter_factors(img, [(0, [0.2, 0.8], -1), (1, [0.3, 0.6, 0.1], 1), (0, [0.2, 0.8], -1)], obord(b))
+        assert relerr(out, want) <= TOL, b
+    res = IF.imfilter(img, kern, IF.Inner())
+    lo = (2, 0)
+    hi = (0, 3)
+    assert res.first == lo and res.data.shape == (40 - 2, 33 - 3)
+    want = O.imfilter_factors(img, [(0, [0.2, 0.8], -1), (1, [0.3, 0.6, 0.1], 1), (0, [0.2, 0.8], -1)], "replicate",
+                              ofirst=list(lo), osize=list(res.data.shape))
+    assert relerr(res.data, want) <= TOL
+
+
+def test_separable_3d_and_1d():
+    rng = np.random.default_rng(14)
+    vol = rng.random((31, 20, 17)).astype(np.float32)
+    kern = KernelFactors.gaussian((2.0, 2.0, 1.5))
+    for b in ["symmetric", "circular", IF.Fill(0.0)]:
+        out = IF.imfilter(vol, kern, b)
+        want = O.imfilter_factors(vol.astype(np.float64), factors_of(kern), obord(b))
+        assert relerr(out, want) <= TOL, b
+    v = rng.random(100)
+    k = IF.centered(np.array([0.25, 0.5, 0.25]))
+    out = IF.imfilter(v, k)
+    want = O.imfilter_factors(v, [(0, [0.25, 0.5, 0.25], -1)], "replicate")
+    assert relerr(out, want) <= TOL
+
+
+def test_row_major_input_same_as_column_major():
+    rng = np.random.default_rng(15)
+    img = rng.random((50, 70)).astype(np.float32)
+    kern = KernelFactors.gaussian((1.5, 2.5))
+    a = IF.imfilter(np.ascontiguousarray(img), kern, "reflect")
+    b = IF.imfilter(np.asfortranarray(img), kern, "reflect")
+    np.testing.assert_allclose(a, b, rtol=1e-6, atol=1e-7)
+
+
+def test_prewitt_tiling_uint8():
+    m = np.zeros((20, 20), dtype=np.uint8)
+    for i in range(-2, 3):
+        idx = np.arange(max(0, -i), min(20, 20 - i))
+        m[idx, idx + i] = 255
+    kernel = KernelFactors.prewitt((True, True), 0)
+    mf = IF.imfilter(m, kernel)
+    dv = [42.5, 85, 85, 42.5]
+    target = np.zeros((20, 20))
+    for i in range(1, 5):
+        target += np.diag(np.full(20 - i, dv[i - 1]), i)
+        target -= np.diag(np.full(20 - i, dv[i - 1]), -i)
+    np.testing.assert_allclose(mf[1:19, 1:19], target[1:19, 1:19], atol=1e-3)
+
+
+# ---- dense FIR ------------------------------------------------------------------------------
+@pytest.mark.parametrize("ks", [(3, 3), (5, 7), (1, 9), (8, 6)])
+def test_dense_valu_small_kernels(ks):
+    rng = np.random.default_rng(16)
+    img = rng.random((45, 38)).astype(np.float32)
+    k = rng.standard_normal(ks)
+    kern = IF.centered(k)
+    for b in STYLES + [IF.Fill(0.5)]:
+        out = IF.imfilter(img, kern, b, IF.Algorithm.FIR())
+        want = O.imfilter_dense(img.astype(np.float64), k, list(kern.first), obord(b))
+        assert relerr(out, want) <= TOL, (ks, b)
+
+
+def norm_random_kernel(rng, K):
+    while True:
+        k = rng.standard_normal((K, K))
+        s = k.sum()
+        if abs(s) >= 1:
+            return k / s
+
+
+@pytest.mark.parametrize("K", [15, 17, 31])
+def test_dense_mfma_split_bf16(K):
+    rng = np.random.default_rng(17 + K)
+    img = rng.random((300, 257)).astype(np.float32)
+    k = norm_random_kernel(rng, K)
+    kern = IF.centered(k)
+    for b in ["reflect", "replicate", "circular", IF.Fill(0.0)]:
+        out = IF.imfilter(img, kern, b, IF.Algorithm.FIR())
+        want = O.imfilter_dense(img.astype(np.float64), k, list(kern.first), obord(b))
+        e = relerr(out, want)
+        assert e <= TOL, (K, b, e)
+
+
+def test_dense_mfma_bf16_storage_and_log():
+    rng = np.random.default_rng(18)
+    x = rng.random((200, 190)).astype(np.float32)
+    xb = IF.bf16(x)
+    xr = IF.bf16_to_float32(xb).astype(np.float64)
+    kern = Kernel.LoG(2.0)
+    out = IF.imfilter(IF.Float32, xb, kern, "reflect", IF.Algorithm.FIR())
+    want = O.imfilter_dense(xr, kern.data, list(kern.first), "reflect")
+    assert relerr(out, want) <= TOL
+    out64 = IF.imfilter(x, kern, "reflect", IF.Algorithm.FIR())
+    assert out64.dtype == np.float64
+    want = O.imfilter_dense(x.astype(np.float64), kern.data, list(kern.first), "reflect")
+    assert relerr(out64, want) <= TOL
+
+
+def test_dense_3d_box_closed_form():
+    img = np.ones((10, 10, 10))
+    kern = IF.centered(np.full((3, 3, 3), 1 / 27))
+    for b in STYLES:
+        np.testing.assert_allclose(IF.imfilter(img, kern, b), img, rtol=1e-6)
+    edge = np.array([2 / 3] + [1.0] * 8 + [2 / 3])
+    want = edge[:, None, None] * edge[None, :, None] * edge[None, None, :]
+    np.testing.assert_allclose(IF.imfilter(img, kern, IF.Fill(0)), want, rtol=1e-6)
+    np.testing.assert_allclose(IF.imfilter(img, kern, IF.Fill(0), IF.Algorithm.FFT()), want, rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("border", STYLES + ["fill"])
+def test_dense_edge_targets(border):
+    kern = np.array([[0.1, 0.2], [0.4, 0.5]])
+    kernel = IF.OffsetArray(kern, range(-1, 1), range(1, 3))
+    img = np.zeros((5, 7))
+    img[0, 1] = 1
+    b = IF.Fill(0.0) if border == "fill" else border
+    want = O.imfilter_dense(img, kern, [-1, 1], obord(b))
+    for alg in (IF.Algorithm.FIR(), IF.Algorithm.FIRTiled(), IF.Algorithm.FFT()):
+        np.testing.assert_allclose(IF.imfilter(img, kernel, b, alg), want, atol=1e-6)
+
+
+# ---- FFT -------------------------------------------------------------------------------------
+@pytest.mark.parametrize("K", [15, 31])
+def test_fft_matches_fir(K):
+    rng = np.random.default_rng(19)
+    img = rng.random((211, 187)).astype(np.float32)
+    k = norm_random_kernel(rng, K)
+    kern = IF.centered(k)
+    for b in ["reflect", "symmetric", "circular", "replicate", IF.Fill(0.25)]:
+        out = IF.imfilter(img, kern, b, IF.Algorithm.FFT())
+        want = O.imfilter_dense(img.astype(np.float64), k, list(kern.first), obord(b))
+        assert relerr(out, want) <= TOL, (K, b)
+    # the default algorithm for a 15x15 kernel is FFT (maxlen > 30)
+    assert IF.filter_algorithm(img, IF.factorkernel(kern)) == IF.Algorithm.FFT()
+
+
+def test_fft_1d_3d():
+    rng = np.random.default_rng(20)
+    v = rng.random(500)
+    k = IF.centered(np.ones(31) / 31)
+    out = IF.imfilter(v, k)            # FFT by default
+    want = O.imfilter_dense(v, np.ones(31) / 31, [-15], "replicate")
+    assert relerr(out, want) <= TOL
+    vol = rng.random((24, 20, 18)).astype(np.float32)
+    k3 = rng.standard_normal((5, 3, 7))
+    out = IF.imfilter(vol, IF.centered(k3), "symmetric", IF.Algorithm.FFT())
+    want = O.imfilter_dense(vol.astype(np.float64), k3, [-2, -1, -3], "symmetric")
+    assert relerr(out, want) <= TOL
+
+
+# ---- IIR -------------------------------------------------------------------------------------
+def iir_spec(kern):
+    return [(f.Npre, f.data.a, f.data.b, f.data.scale, f.data.M) for f in kern]
+
+
+@pytest.mark.parametrize("border", ["replicate", "circular", "symmetric", IF.Fill(0.0), IF.Fill(0.7)])
+def test_iir_matches_oracle(border):
+    rng = np.random.default_rng(21)
+    vol = (rng.random((40, 33, 21)) + rng.normal(0, 0.05, (40, 33, 21))).astype(np.float32)
+    kern = KernelFactors.IIRGaussian((2.0, 2.0, 1.5))
+    out = IF.imfilter(vol, kern, border)
+    assert out.dtype == np.float64
+    # any Pad style initialises from the edge pixel (src/imfilter.jl:1030-1032)
+    ob = ("fill", border.value) if isinstance(border, IF.Fill) else "replicate"
+    want = O.imfilter_iir(vol.astype(np.float64), iir_spec(kern), ob)
+    np.testing.assert_allclose(out, want, rtol=1e-9, atol=1e-12)
+
+
+def test_iir_per_dim_call_and_errors():
+    imgf = np.zeros((5, 7))
+    imgf[2, 3] = 1
+    ret = IF.imfilter(imgf, KernelFactors.IIRGaussian((0, 5)), "replicate")
+    out = np.zeros_like(ret)
+    IF.imfilter_(IF.CPU1(IF.Algorithm.IIR()), out, imgf, KernelFactors.IIRGaussian(5), 2, "replicate")
+    np.testing.assert_array_equal(out, ret)
+    with pytest.raises(IF.DimensionMismatch):
+        IF.imfilter(np.array([1.0, 2.0]), (KernelFactors.IIRGaussian(2.0),))
+
+
+# ---- median / min / max / extrema -------------------------------------------------------------
+@pytest.mark.parametrize("w", [3, 5, 7])
+def test_median_u8_n0f8_bit_exact(w):
+    rng = np.random.default_rng(22)
+    codes = rng.integers(0, 256, (67, 93)).astype(np.uint8)
+    sp = rng.random(codes.shape)
+    codes[sp < 0.025] = 0
+    codes[sp > 0.975] = 255
+    h = w // 2
+    for b in STYLES + [IF.Fill(0)]:
+        want = O.median(codes.astype(np.float64), [-h, -h], [w, w], obord(b))
+        got = IF.mapwindow(IF.median_, codes, (w, w), border=b)
+        assert got.dtype == np.float64
+        np.testing.assert_array_equal(got, want, err_msg=str(b))
+        gotn = IF.mapwindow(IF.median_, IF.n0f8(codes), (w, w), border=b)
+        assert gotn.dtype == np.float32
+        np.testing.assert_array_equal(gotn, (want / 255).astype(np.float32))
+
+
+def test_median_float_isless_and_generic_windows():
+    rng = np.random.default_rng(23)
+    a = rng.standard_normal((31, 26)).astype(np.float32)
+    a[3, 4] = np.nan
+    a[10, 10] = -0.0
+    a[10, 11] = 0.0
+    for w in [(3, 3), (5, 5), (7, 7), (2, 2), (1, 5), (4, 3)]:
+        win = tuple(range(-(x // 2), x - x // 2) for x in w)
+        for b in STYLES + [IF.Fill(0.5)]:
+            got = IF.mapwindow(IF.median_, a, win, border=b)
+            want = O.median(a.astype(np.float64), [r.start for r in win], list(w), obord(b))
+            np.testing.assert_array_equal(got.view(np.uint32) if False else got, want.astype(np.float32),
+                                          err_msg="%r %r" % (w, b))
+            assert np.array_equal(np.signbit(got), np.signbit(want.astype(np.float32)))
+    v = rng.standard_normal(40)
+    np.testing.assert_array_equal(IF.mapwindow(IF.median_, v, 5), O.median(v, [-2], [5], "replicate"))
+
+
+def test_median_reference_vectors():
+    b = np.array([1, 100, 1, 2, -1000, 2], dtype=np.float64)
+    np.testing.assert_array_equal(IF.mapwindow(IF.median_, b, range(-1, 2)), [1, 1, 2, 1, 2, 2])
+    np.testing.assert_array_equal(IF.mapwindow(IF.median, b, range(-2, 3)), [1, 1, 1, 2, 2, 2])
+    A = np.array([[1, 5, -2, 3, 7], [2, 0, 3, 4, 4], [3, 3, 6, 2, 5], [1, -3, 5, 3, 0]], dtype=np.float64)
+    np.testing.assert_array_equal(IF.mapwindow(IF.median_, A, (3, 3)),
+                                  [[1, 1, 3, 3, 4], [2, 3, 3, 4, 4], [2, 3, 3, 4, 4], [1, 3, 3, 3, 2]])
+
+
+@pytest.mark.parametrize("w", [3, 9])
+def test_minmax_bit_exact(w):
+    rng = np.random.default_rng(24)
+    codes = rng.integers(0, 256, (71, 53)).astype(np.uint8)
+    h = w // 2
+    for b in ["replicate", "symmetric", "circular", IF.Fill(17)]:
+        mn, mx = O.minmax(codes.astype(np.float64), [-h, -h], [w, w], obord(b))
+        np.testing.assert_array_equal(IF.mapwindow(IF.minimum, codes, (w, w), border=b), mn)
+        np.testing.assert_array_equal(IF.mapwindow(IF.maximum, codes, (w, w), border=b), mx)
+    f = rng.standard_normal((33, 41)).astype(np.float32)
+    mn, mx = O.minmax(f.astype(np.float64), [-h, -h], [w, w], "replicate")
+    np.testing.assert_array_equal(IF.mapwindow(IF.minimum, f, (w, w)), mn.astype(np.float32))
+    np.testing.assert_array_equal(IF.mapwindow(IF.maximum, f, (w, w)), mx.astype(np.float32))
+
+
+def test_extrema_lemire_semantics():
+    import json, os
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_fixtures.json")))
+    for c in g["extrema_1d"]["cases"]:
+        got = IF.mapwindow(IF.extrema, np.array(c["input"]), c["window"])
+        np.testing.assert_array_equal(got, np.array(c["expect"]), err_msg=str(c))
+    rng = np.random.default_rng(25)
+    for window in [(2, 2), (2, 3), (3, 3), (2, 5), (9, 9), (2, 5, 3)]:
+        A = rng.random((11,) * len(window)).astype(np.float32)
+        mn, mx = O.extrema_lemire(A.astype(np.float64), list(window))
+        got = IF.mapwindow(IF.extrema, A, window)
+        np.testing.assert_array_equal(got[..., 0], mn.astype(np.float32))
+        np.testing.assert_array_equal(got[..., 1], mx.astype(np.float32))
+
+
+def test_errors_from_the_library():
+    with pytest.raises(IF.ArgumentError):
+        IF.padarray(np.zeros((1, 4)), IF.Pad("reflect", (1, 1), (1, 1)))
+    with pytest.raises(IF.DimensionMismatch):
+        IF.imfilter(np.zeros((3, 9)), KernelFactors.IIRGaussian((2.0, 2.0)))
+
+
+def test_device_resident_roundtrip():
+    rng = np.random.default_rng(26)
+    img = rng.random((128, 96)).astype(np.float32)
+    d = IF.DeviceArray.from_host(img)
+    kern = KernelFactors.gaussian((2.0, 2.0))
+    out = IF.imfilter(IF.Float32, d, kern, "symmetric")
+    assert isinstance(out, IF.DeviceArray)
+    want = O.imfilter_factors(img.astype(np.float64), factors_of(kern), "symmetric")
+    assert relerr(out.to_host(), want) <= TOL

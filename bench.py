@@ -87,7 +87,7 @@ def main():
     img = rng.random((n, n), dtype=np.float32)
     k15 = norm_kernel(np.random.default_rng(20261214), 15)
     d_img = IF.DeviceArray.from_host(img)
-    d_out = IF.DeviceArray((n, n), IF.Float32)
+    d_out = d_img.similar()
     a_in, a_out = ifm_array(d_img), ifm_array(d_out)
     b_reflect = _border_struct(IF.Pad("reflect"), IF.Float32)
     kern15 = IF.centered(k15)
@@ -203,13 +203,13 @@ def run_extras(IF, _lib, L, ctx, T, img, d_img, d_out, b_reflect, args):
     rate("dense15_valu_fp32_4096", lambda: L.ifm_imfilter_dense(ctx, a_in, a_out, None, ks15, b_reflect, 1), n * n, 8, 3)
     # BF16 storage in and out
     xb = IF.DeviceArray.from_host(IF.bf16(img))
-    ob = IF.DeviceArray((n, n), IF.BFloat16)
+    ob = xb.similar()
     a_b, a_ob = ifm_array(xb), ifm_array(ob)
     ks15 = _kernel_struct(k15)
     rate("dense15_mfma_bf16io_4096", lambda: L.ifm_imfilter_dense(ctx, a_b, a_ob, None, ks15, b_reflect, 0), n * n, 4)
     # config 1: 1024^2 gaussian sigma 3 replicate, F32 -> F64
     c1 = IF.DeviceArray.from_host(np.random.default_rng(1).random((1024, 1024), dtype=np.float32))
-    o1 = IF.DeviceArray((1024, 1024), IF.Float64)
+    o1 = c1.similar(IF.Float64)
     kern = IF.KernelFactors.gaussian((3.0, 3.0))
     fa = (_lib.IfmFactor * 2)()
     keep = []

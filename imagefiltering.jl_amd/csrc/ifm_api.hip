@@ -729,11 +729,28 @@ static int window_setup(ifm_ctx *ctx, const Geo &g, const int64_t *wf, const int
   return IFM_OK;
 }
 
-static int window_reflect_check(ifm_ctx *ctx, int style, const Dense4 &in, const int *wf, const int *wl) {
-  if (style != IFM_REFLECT) return IFM_OK;
-  for (int m = 0; m < MAXD; ++m)
-    if (in.n[m] == 1 && (wf[m] != 0 || wl[m] != 1))
-      return fail(ctx, IFM_E_ARG, "reflect padding of an axis of length 1 (zero period)");
+// mapwindow maps edge windows relative to their intersection with the image
+// (padindex(border, inner, outer), src/mapwindow.jl:310-316): an empty
+// intersection fails for every Pad style, and reflect's period 2n-2 is zero
+// for an intersection of one element (the reference's DivideError).
+static int window_edge_check(ifm_ctx *ctx, int style, const Dense4 &in, const Dense4 &out, const int64_t *of,
+                             const int *wf, const int *wl) {
+  if (style == IFM_FILL) return IFM_OK;
+  for (int m = 0; m < MAXD; ++m) {
+    if (wf[m] == 0 && wl[m] == 1) continue;
+    const int64_t n = in.n[m];
+    auto bad = [&](int64_t q) {
+      int64_t lo = q + wf[m], hi = q + wf[m] + wl[m] - 1;
+      if (lo >= 0 && hi <= n - 1) return false;
+      int64_t len = std::min(hi, n - 1) - std::max(lo, (int64_t)0) + 1;
+      return len <= 0 || (style == IFM_REFLECT && len == 1);
+    };
+    const int64_t q0 = of[m], q1 = of[m] + out.n[m];
+    for (int64_t q = q0; q < std::min(q1, q0 + wl[m] + 1); ++q)
+      if (bad(q)) return fail(ctx, IFM_E_ARG, "window does not overlap the image enough for this border (empty or zero-period index map)");
+    for (int64_t q = std::max(q0, q1 - wl[m] - 1); q < q1; ++q)
+      if (bad(q)) return fail(ctx, IFM_E_ARG, "window does not overlap the image enough for this border (empty or zero-period index map)");
+  }
   return IFM_OK;
 }
 
@@ -755,7 +772,7 @@ int ifm_mapwindow_median(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, cons
   int64_t of[MAXD];
   to_canon(g, out_first, 0, of);
   if ((rc = check_region(ctx, vi.d, vo.d, of))) return rc;
-  if ((rc = window_reflect_check(ctx, b->style, vi.d, wf, wl))) return rc;
+  if ((rc = window_edge_check(ctx, b->style, vi.d, vo.d, of, wf, wl))) return rc;
   if (vi.d.count() == 0 && vo.d.count() > 0) return fail(ctx, IFM_E_ARG, "empty image");
   HIPCHK(launch_median(vi.d, vo.d, of, wf, wl, b->style, b->fill, ctx->stream));
   if ((rc = finish_view(ctx, vo))) return rc;
@@ -789,7 +806,7 @@ static int minmax_common(ifm_ctx *ctx, const ifm_array *in, ifm_array *omin, ifm
   to_canon(g, out_first, 0, of);
   Dense4 &r = omin ? vmin.d : vmax.d;
   if ((rc = check_region(ctx, vi.d, r, of))) return rc;
-  if (!lemire && (rc = window_reflect_check(ctx, style, vi.d, wf, wl))) return rc;
+  if (!lemire && (rc = window_edge_check(ctx, style, vi.d, r, of, wf, wl))) return rc;
   if (vi.d.count() == 0 && r.count() > 0) return fail(ctx, IFM_E_ARG, "empty image");
   size_t ws = minmax_workspace(vi.d, r.n);
   void *work = scratch(ctx, S_WORK, ws);

@@ -51,12 +51,11 @@ def n0f8(codes):
 
 def bf16(values):
     """Round float values to BFloat16 (nearest even) and tag the bits."""
-    a = np.asarray(values, dtype=np.float32)
+    a = np.array(values, dtype=np.float32, ndmin=1)
+    shape = np.shape(values)
     u = a.view(np.uint32).astype(np.uint64)
-    nan = np.isnan(a)
-    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
-    r[nan] = ((u[nan] >> 16) | 0x40).astype(np.uint16)
-    return r.view(BF16Array)
+    r = np.where(np.isnan(a), (u >> 16) | 0x40, (u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+    return r.reshape(shape).view(BF16Array)
 
 
 def bf16_to_float32(bits):

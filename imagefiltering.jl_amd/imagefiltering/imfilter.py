@@ -310,8 +310,10 @@ def _run(alg, out, img, kernel, border, out_first):
     if isinstance(alg, IIR) or kinds == {"iir"}:
         if kinds != {"iir"}:
             raise NotImplementedError("IIR algorithm with FIR factors (Algorithm.Mixed) is not implemented")
-        if isinstance(border, Pad) and border.lo is not None and any(border.lo + border.hi):
-            pass
+        if isinstance(border, Fill):
+            # leftborder!/rightborder! convert the Fill value to the eltype of
+            # the padded array, i.e. the output eltype (src/imfilter.jl:1027-1029)
+            b.fill = convert_fill(border.value, eltype_of(out))
         arr = (_lib.IfmIir * len(comps))()
         for i, c in enumerate(comps):
             k = c.iir

@@ -336,6 +336,8 @@ static int cmpkey(const void *x, const void *y) {
  * the image are copied; at the edge, copy_win! maps every axis with
  * padindex(border, inner, outer) where inner is the window's intersection
  * with the image axes (:310-316), or fills with the Fill value (:323-331). */
+static int g_window_err; /* set when an edge window has an empty or zero-period index map */
+
 static int64_t gather_window(const double *img, const int64_t *N, int style, double fill,
                              const int64_t *q, const int64_t *WF, const int64_t *WL, double *buf) {
   int64_t ilo[MAXD], ilen[MAXD];
@@ -355,7 +357,8 @@ static int64_t gather_window(const double *img, const int64_t *N, int style, dou
           for (int d = 0; d < MAXD; ++d) {
             if (x[d] >= 0 && x[d] < N[d]) { s[d] = x[d]; continue; }
             outside = 1;
-            if (style == 4 || ilen[d] <= 0) { s[d] = -1; continue; }
+            if (style == 4) { s[d] = -1; continue; }
+            if (ilen[d] <= 0 || (style == 3 && ilen[d] == 1)) { g_window_err = 1; s[d] = -1; continue; }
             s[d] = ilo[d] + orc_padmap(style, x[d] - ilo[d], ilen[d]);
           }
           int isfill = 0;
@@ -384,6 +387,7 @@ int orc_median_f64(const double *img, int ndim, const int64_t *n, const int64_t 
   }
   double *buf = (double *)malloc(sizeof(double) * (size_t)L);
   uint64_t *key = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)L);
+  g_window_err = 0;
   int64_t p = 0, q[MAXD];
   for (int64_t o3 = 0; o3 < OS[3]; ++o3)
     for (int64_t o2 = 0; o2 < OS[2]; ++o2)
@@ -404,7 +408,7 @@ int orc_median_f64(const double *img, int ndim, const int64_t *n, const int64_t 
           out[p] = v;
         }
   free(buf); free(key);
-  return 0;
+  return g_window_err ? -1 : 0;
 }
 
 /* Julia min/max on floats: NaN-propagating, -0.0 < +0.0. */
@@ -435,6 +439,7 @@ int orc_minmax_f64(const double *img, int ndim, const int64_t *n, const int64_t 
     L *= WL[d];
   }
   double *buf = (double *)malloc(sizeof(double) * (size_t)L);
+  g_window_err = 0;
   int64_t p = 0, q[MAXD];
   for (int64_t o3 = 0; o3 < OS[3]; ++o3)
     for (int64_t o2 = 0; o2 < OS[2]; ++o2)
@@ -448,7 +453,7 @@ int orc_minmax_f64(const double *img, int ndim, const int64_t *n, const int64_t 
           if (omax) omax[p] = mx;
         }
   free(buf);
-  return 0;
+  return g_window_err ? -1 : 0;
 }
 
 /* Lemire streaming min/max along axis 0 of (mn, mx) pairs

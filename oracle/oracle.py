@@ -153,8 +153,9 @@ def median(img, wfirst, wlen, border, ofirst=None, osize=None):
     s, fill = _style(border)
     of, os_ = _region(a, ofirst, osize)
     o, op = _outbuf(tuple(os_))
-    lib().orc_median_f64(ap, a.ndim, _i64(a.shape), _i64(wfirst), _i64(wlen), s, fill, _i64(of),
-                         _i64(os_), op)
+    if lib().orc_median_f64(ap, a.ndim, _i64(a.shape), _i64(wfirst), _i64(wlen), s, fill, _i64(of),
+                            _i64(os_), op) != 0:
+        raise ZeroDivisionError("edge window with an empty or zero-period index map")
     return o
 
 
@@ -164,8 +165,9 @@ def minmax(img, wfirst, wlen, border, ofirst=None, osize=None):
     of, os_ = _region(a, ofirst, osize)
     omin, pmin = _outbuf(tuple(os_))
     omax, pmax = _outbuf(tuple(os_))
-    lib().orc_minmax_f64(ap, a.ndim, _i64(a.shape), _i64(wfirst), _i64(wlen), s, fill, _i64(of),
-                         _i64(os_), pmin, pmax)
+    if lib().orc_minmax_f64(ap, a.ndim, _i64(a.shape), _i64(wfirst), _i64(wlen), s, fill, _i64(of),
+                            _i64(os_), pmin, pmax) != 0:
+        raise ZeroDivisionError("edge window with an empty or zero-period index map")
     return omin, omax
 
 

@@ -333,9 +333,15 @@ def test_median_float_isless_and_generic_windows():
     for w in [(3, 3), (5, 5), (7, 7), (2, 2), (1, 5), (4, 3)]:
         win = tuple(range(-(x // 2), x - x // 2) for x in w)
         for b in STYLES + [IF.Fill(0.5)]:
+            try:
+                want = O.median(a.astype(np.float64), [r.start for r in win], list(w), obord(b))
+            except ZeroDivisionError:
+                # reflect over a one-element window/image overlap: the reference's DivideError
+                with pytest.raises(IF.ArgumentError):
+                    IF.mapwindow(IF.median_, a, win, border=b)
+                continue
             got = IF.mapwindow(IF.median_, a, win, border=b)
-            want = O.median(a.astype(np.float64), [r.start for r in win], list(w), obord(b))
-            np.testing.assert_array_equal(got.view(np.uint32) if False else got, want.astype(np.float32),
+            np.testing.assert_array_equal(got, want.astype(np.float32),
                                           err_msg="%r %r" % (w, b))
             assert np.array_equal(np.signbit(got), np.signbit(want.astype(np.float32)))
     v = rng.standard_normal(40)

@@ -23,7 +23,10 @@ struct ifm_ctx {
   struct TapEntry {
     uint64_t hash;
     size_t n;
-    void *dev;
+    void *dev;                // FP32 taps
+    void *table = nullptr;    // MFMA Toeplitz fragments (built on first MFMA use)
+    float kinv = 1.f;
+    int kx = 0, ky = 0;
   };
   std::vector<TapEntry> taps;
 };
@@ -241,7 +244,10 @@ int ifm_ctx_destroy(ifm_ctx *ctx) {
     if (s.first) (void)hipFree(s.first);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
-  for (auto &e : ctx->taps) (void)hipFree(e.dev);
+  for (auto &e : ctx->taps) {
+    (void)hipFree(e.dev);
+    if (e.table) (void)hipFree(e.table);
+  }
   delete ctx;
   return IFM_OK;
 }
@@ -470,25 +476,43 @@ int ifm_imfilter_fir(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const in
 }
 
 // ---- FIR, dense --------------------------------------------------------------------
-static int device_taps(ifm_ctx *ctx, const std::vector<float> &t, const float **out) {
-  uint64_t h = 1469598103934665603ull;  // FNV-1a over the tap bytes
+static int device_taps(ifm_ctx *ctx, const DenseKernel &dk, bool want_table, ifm_ctx::TapEntry **out) {
+  const std::vector<float> &t = dk.taps;
+  uint64_t h = 1469598103934665603ull;  // FNV-1a over the tap bytes and the kernel shape
   const unsigned char *p = (const unsigned char *)t.data();
   for (size_t i = 0; i < t.size() * sizeof(float); ++i) h = (h ^ p[i]) * 1099511628211ull;
+  for (int d = 0; d < MAXD; ++d) h = (h ^ (uint64_t)dk.n[d]) * 1099511628211ull;
+  ifm_ctx::TapEntry *hit = nullptr;
   for (auto &e : ctx->taps)
-    if (e.hash == h && e.n == t.size()) {
-      *out = (const float *)e.dev;
-      return IFM_OK;
+    if (e.hash == h && e.n == t.size()) hit = &e;
+  if (!hit) {
+    if (ctx->taps.size() >= 64) {  // bounded cache: drop the oldest entry
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+      HIPCHK(hipFree(ctx->taps.front().dev));
+      if (ctx->taps.front().table) HIPCHK(hipFree(ctx->taps.front().table));
+      ctx->taps.erase(ctx->taps.begin());
     }
-  if (ctx->taps.size() >= 64) {  // bounded cache: drop the oldest entry
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    HIPCHK(hipFree(ctx->taps.front().dev));
-    ctx->taps.erase(ctx->taps.begin());
+    void *d = nullptr;
+    if (hipMalloc(&d, t.size() * sizeof(float)) != hipSuccess)
+      return fail(ctx, IFM_E_NOMEM, "device allocation failed (taps)");
+    HIPCHK(hipMemcpy(d, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
+    ifm_ctx::TapEntry e;
+    e.hash = h;
+    e.n = t.size();
+    e.dev = d;
+    ctx->taps.push_back(e);
+    hit = &ctx->taps.back();
   }
-  void *d = nullptr;
-  if (hipMalloc(&d, t.size() * sizeof(float)) != hipSuccess) return fail(ctx, IFM_E_NOMEM, "device allocation failed (taps)");
-  HIPCHK(hipMemcpy(d, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
-  ctx->taps.push_back({h, t.size(), d});
-  *out = (const float *)d;
+  if (want_table && !hit->table) {
+    std::vector<uint16_t> tab;
+    dense_mfma_table(dk, tab, &hit->kinv);
+    if (hipMalloc(&hit->table, tab.size() * sizeof(uint16_t)) != hipSuccess) {
+      hit->table = nullptr;
+      return fail(ctx, IFM_E_NOMEM, "device allocation failed (Toeplitz table)");
+    }
+    HIPCHK(hipMemcpy(hit->table, tab.data(), tab.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+  }
+  *out = hit;
   return IFM_OK;
 }
 
@@ -555,13 +579,13 @@ int ifm_imfilter_dense(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const 
   else if (precision == IFM_PREC_AUTO)
     mfma = twod && dk.n[0] >= 15 && dk.n[1] >= 15 && dense_mfma_supported(dk, vi.d);
   else if (precision != IFM_PREC_FP32) return fail(ctx, IFM_E_ARG, "unknown precision");
-  const float *dtaps = nullptr;
-  if ((rc = device_taps(ctx, dk.taps, &dtaps))) return rc;
+  ifm_ctx::TapEntry *te = nullptr;
+  if ((rc = device_taps(ctx, dk, mfma, &te))) return rc;
   if (mfma)
-    HIPCHK(launch_dense_mfma(vi.d, vo.d, of, dk, (void *)dtaps, dk.taps.size() * sizeof(float), b->style, (float)b->fill,
-                             ctx->stream));
+    HIPCHK(launch_dense_mfma(vi.d, vo.d, of, dk, (const uint16_t *)te->table, (const float *)te->dev, te->kinv,
+                             b->style, (float)b->fill, ctx->stream));
   else
-    HIPCHK(launch_dense_valu(vi.d, vo.d, of, dk, dtaps, b->style, (float)b->fill, ctx->stream));
+    HIPCHK(launch_dense_valu(vi.d, vo.d, of, dk, (const float *)te->dev, b->style, (float)b->fill, ctx->stream));
   if ((rc = finish_view(ctx, vo))) return rc;
   return sync_if_host(ctx, !in->device || !out->device);
 }

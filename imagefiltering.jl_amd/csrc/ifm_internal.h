@@ -188,11 +188,12 @@ struct DenseKernel {
 };
 hipError_t launch_dense_valu(const Dense4 &in, Dense4 &out, const int64_t *out_first, const DenseKernel &k,
                              const float *d_taps, int style, float fill, hipStream_t st);
-// MFMA banded-Toeplitz path (2-D, axis 0 and 1 extents up to 64).
+// MFMA banded-Toeplitz path (2-D, kernels up to 64 x 64).
 bool dense_mfma_supported(const DenseKernel &k, const Dense4 &in);
+void dense_mfma_table(const DenseKernel &k, std::vector<uint16_t> &table, float *kinv);
 hipError_t launch_dense_mfma(const Dense4 &in, Dense4 &out, const int64_t *out_first, const DenseKernel &k,
-                             void *d_work, size_t work_bytes, int style, float fill, hipStream_t st);
-size_t dense_mfma_workspace(const DenseKernel &k);
+                             const uint16_t *d_table, const float *d_taps, float kinv, int style, float fill,
+                             hipStream_t st);
 
 // IIR: one cascade stage along canonical axis `axis`, FP64 recursion.
 struct IirCoef {

@@ -122,27 +122,43 @@ int orc_imfilter_dense(const double *img, int ndim, const int64_t *n, const doub
     OS[d] = d < ndim ? (osize ? osize[d] : N[d]) : 1;
   }
   for (int d = 0; d < ndim; ++d)
-    if (style == 3 && N[d] == 1) return -1;
-  /* Materialise the padded rows the way padarray does, but lazily per output. */
-  int64_t p = 0, x[MAXD];
+    if (style == 3 && N[d] == 1 && (KS[d] > 1 || KF[d] != 0)) return -1;
+  /* padarray: materialise the padded block the outputs need
+   * (src/imfilter.jl:336), then the direct correlation loop. */
+  int64_t P[MAXD], PO[MAXD], PS[MAXD], cnt = 1;
+  for (int d = 0; d < MAXD; ++d) {
+    P[d] = OS[d] + KS[d] - 1;
+    PO[d] = OF[d] + KF[d];  /* image coordinate of padded element 0 */
+    PS[d] = cnt;
+    cnt *= P[d];
+  }
+  double *pad = (double *)malloc(sizeof(double) * (size_t)(cnt ? cnt : 1));
+  int64_t x[MAXD], p = 0;
+  for (int64_t i3 = 0; i3 < P[3]; ++i3)
+    for (int64_t i2 = 0; i2 < P[2]; ++i2)
+      for (int64_t i1 = 0; i1 < P[1]; ++i1)
+        for (int64_t i0 = 0; i0 < P[0]; ++i0, ++p) {
+          x[0] = PO[0] + i0; x[1] = PO[1] + i1; x[2] = PO[2] + i2; x[3] = PO[3] + i3;
+          pad[p] = padval(img, N, style, fill, x);
+        }
+  /* __imfilter_inbounds!: tmp += A[i+j, I+J] * k[j, J], j (dim 1) innermost */
+  p = 0;
   for (int64_t o3 = 0; o3 < OS[3]; ++o3)
     for (int64_t o2 = 0; o2 < OS[2]; ++o2)
       for (int64_t o1 = 0; o1 < OS[1]; ++o1)
         for (int64_t o0 = 0; o0 < OS[0]; ++o0, ++p) {
           double tmp = 0.0;
-          int64_t q = 0;
+          const double *kq = k;
           for (int64_t j3 = 0; j3 < KS[3]; ++j3)
             for (int64_t j2 = 0; j2 < KS[2]; ++j2)
-              for (int64_t j1 = 0; j1 < KS[1]; ++j1)
-                for (int64_t j0 = 0; j0 < KS[0]; ++j0, ++q) {
-                  x[0] = OF[0] + o0 + KF[0] + j0;
-                  x[1] = OF[1] + o1 + KF[1] + j1;
-                  x[2] = OF[2] + o2 + KF[2] + j2;
-                  x[3] = OF[3] + o3 + KF[3] + j3;
-                  tmp += padval(img, N, style, fill, x) * k[q];
-                }
+              for (int64_t j1 = 0; j1 < KS[1]; ++j1) {
+                const double *row = pad + o0 + (o1 + j1) * PS[1] + (o2 + j2) * PS[2] + (o3 + j3) * PS[3];
+                for (int64_t j0 = 0; j0 < KS[0]; ++j0) tmp += row[j0] * kq[j0];
+                kq += KS[0];
+              }
           out[p] = tmp;
         }
+  free(pad);
   return 0;
 }
 

@@ -222,6 +222,56 @@ def run_extras(IF, _lib, L, ctx, T, img, d_img, d_out, b_reflect, args):
     a1, ao1 = ifm_array(c1), ifm_array(o1)
     brep = _border_struct(IF.Pad("replicate"), IF.Float32)
     rate("gauss13_sep_1024_f64out", lambda: L.ifm_imfilter_fir(ctx, a1, ao1, None, fa, 2, brep), 1024 * 1024, 12, 20)
+    del c1, o1
+
+    from imagefiltering.imfilter import _processed, _run
+    from imagefiltering import Kernel, KernelFactors
+
+    def api(alg, out, x, kernel, border):
+        comps = _processed(kernel)  # factoring (SVD) happens once, outside the timed loop
+        return lambda: _run(alg, out, x, comps, IF.borderinstance(border), None)
+
+    # sobel (SVD-factored to 3 + 3 taps) and LoG(2.0) (17 x 17, matrix cores) on the 4096^2 image, F64 out
+    o64 = d_img.similar(IF.Float64)
+    rate("sobel1_4096_f64out", api(IF.Algorithm.FIR(), o64, d_img, Kernel.sobel()[0], "replicate"), n * n, 12)
+    rate("log2_17x17_4096_f64out", api(IF.Algorithm.FIR(), o64, d_img, Kernel.LoG(2.0), "replicate"), n * n, 12)
+    del o64
+
+    # 512 x 512 x 256 volume: separable FIR (symmetric, circular) and IIR (circular), F64 out
+    vrng = np.random.default_rng(20261205)
+    vol = (vrng.random((512, 512, 256), dtype=np.float32)
+           + vrng.normal(0, 0.05, (512, 512, 256)).astype(np.float32))
+    d_vol = IF.DeviceArray.from_host(vol)
+    del vol
+    o_vol = d_vol.similar(IF.Float64)
+    nv = 512 * 512 * 256
+    g3 = KernelFactors.gaussian((2.0, 2.0, 1.5))
+    rate("gauss_sep3d_512x512x256_symmetric", api(IF.Algorithm.FIRTiled(), o_vol, d_vol, g3, "symmetric"), nv, 12, 3)
+    rate("gauss_sep3d_512x512x256_circular", api(IF.Algorithm.FIRTiled(), o_vol, d_vol, g3, "circular"), nv, 12, 3)
+    i3 = KernelFactors.IIRGaussian((2.0, 2.0, 1.5))
+    rate("iir_gauss3d_512x512x256_circular", api(IF.Algorithm.IIR(), o_vol, d_vol, i3, "circular"), nv, 12, 3)
+    del d_vol, o_vol
+
+    # 8192^2 N0f8 with 5% salt and pepper: median 5x5 / 7x7, extrema 3x3 / 9x9, minimum 9x9
+    crng = np.random.default_rng(20261206)
+    codes = crng.integers(0, 256, (8192, 8192), dtype=np.uint8)
+    u = crng.random((8192, 8192))
+    codes[u < 0.025] = 0
+    codes[(u >= 0.025) & (u < 0.05)] = 255
+    d_c = IF.DeviceArray.from_host(IF.n0f8(codes) if hasattr(IF, "n0f8") else codes)
+    del codes, u
+    oa, ob = d_c.similar(), d_c.similar()
+    ac, aa, abb = ifm_array(d_c), ifm_array(oa), ifm_array(ob)
+    npx = 8192 * 8192
+    for w in (5, 7):
+        wf, wl = _lib.i64([-(w // 2)] * 2), _lib.i64([w] * 2)
+        rate("median%dx%d_8192_n0f8" % (w, w),
+             lambda wf=wf, wl=wl: L.ifm_mapwindow_median(ctx, ac, aa, None, wf, wl, brep), npx, 2, 5)
+    for w in (3, 9):
+        win = _lib.i64([w, w])
+        rate("extrema%dx%d_8192_n0f8" % (w, w), lambda win=win: L.ifm_extrema_filter(ctx, ac, aa, abb, win), npx, 3, 5)
+    wf, wl = _lib.i64([-4, -4]), _lib.i64([9, 9])
+    rate("minimum9x9_8192_n0f8", lambda: L.ifm_mapwindow_minmax(ctx, ac, aa, None, None, wf, wl, brep), npx, 2, 5)
     return ex
 
 

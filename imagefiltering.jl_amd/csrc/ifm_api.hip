@@ -12,25 +12,6 @@
 
 using namespace ifm;
 
-struct ifm_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  std::string err;
-  std::vector<std::pair<void *, size_t>> slots;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  // Device copies of dense kernels keyed by a hash of their FP32 taps, so a
-  // kernel used call after call is uploaded once (no per-call copy or sync).
-  struct TapEntry {
-    uint64_t hash;
-    size_t n;
-    void *dev;                // FP32 taps
-    void *table = nullptr;    // MFMA Toeplitz fragments (built on first MFMA use)
-    float kinv = 1.f;
-    int kx = 0, ky = 0;
-  };
-  std::vector<TapEntry> taps;
-};
-
 namespace {
 
 enum Slot { S_IN = 0, S_OUT, S_OUT2, S_TMP1, S_TMP2, S_TAPS, S_WORK, S_COUNT };
@@ -248,6 +229,7 @@ int ifm_ctx_destroy(ifm_ctx *ctx) {
     (void)hipFree(e.dev);
     if (e.table) (void)hipFree(e.table);
   }
+  batch_release(ctx);
   delete ctx;
   return IFM_OK;
 }
@@ -575,8 +557,10 @@ int ifm_imfilter_dense(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const 
   if ((rc = check_reflect(ctx, b, vi.d, padded))) return rc;
   bool twod = dk.n[2] == 1 && dk.n[3] == 1 && dk.first[2] == 0 && dk.first[3] == 0;
   bool mfma = false;
-  if (precision == IFM_PREC_BF16X3) mfma = twod && dense_mfma_supported(dk, vi.d);
-  else if (precision == IFM_PREC_AUTO)
+  if (precision == IFM_PREC_F16X3) {
+    mfma = twod && dense_mfma_supported(dk, vi.d);
+    if (!mfma) return fail(ctx, IFM_E_UNSUPPORTED, "matrix-core path needs a 2-D kernel of at most 49 x 32 taps");
+  } else if (precision == IFM_PREC_AUTO)
     mfma = twod && dk.n[0] >= 15 && dk.n[1] >= 15 && dense_mfma_supported(dk, vi.d);
   else if (precision != IFM_PREC_FP32) return fail(ctx, IFM_E_ARG, "unknown precision");
   ifm_ctx::TapEntry *te = nullptr;

@@ -188,12 +188,37 @@ struct DenseKernel {
 };
 hipError_t launch_dense_valu(const Dense4 &in, Dense4 &out, const int64_t *out_first, const DenseKernel &k,
                              const float *d_taps, int style, float fill, hipStream_t st);
-// MFMA banded-Toeplitz path (2-D, kernels up to 64 x 64).
+// Matrix-core banded-Toeplitz path (2-D kernels up to 49 x 32).
+struct MfmaArgs {
+  const void *in;
+  void *out;
+  const uint16_t *ttab;  // [chunk][u][hi/lo][lane][8] FP16 Toeplitz fragments
+  const float *taps;     // Kx * Ky FP32 taps (direct path of non-finite tiles)
+  float kinv;            // 1 / kernel scale
+  int n0, n1;            // input plane
+  int o0, o1;            // output plane
+  int f0, f1;            // out_first along axes 0 and 1
+  int kx, ky;
+  int first0, first1;    // kernel first offsets
+  int ntx, nty, ntiles;  // tiles along x, along y, in total over all planes
+  int64_t in_plane, out_plane;
+  int o2, n2, f2, f3;    // plane mapping for axes 2 and 3
+  int style;
+  int dt_in, dt_out;
+  float fill;
+};
 bool dense_mfma_supported(const DenseKernel &k, const Dense4 &in);
 void dense_mfma_table(const DenseKernel &k, std::vector<uint16_t> &table, float *kinv);
+// Arguments of one image (ntiles = its tiles); false if the shape is unsupported.
+bool dense_mfma_args(const Dense4 &in, Dense4 &out, const int64_t *out_first, const DenseKernel &k,
+                     const uint16_t *d_table, const float *d_taps, float kinv, int style, float fill, MfmaArgs *a);
+// Kernel variant of a kernel shape (equal for requests that can share a launch).
+int dense_mfma_variant(const DenseKernel &k);
 hipError_t launch_dense_mfma(const Dense4 &in, Dense4 &out, const int64_t *out_first, const DenseKernel &k,
                              const uint16_t *d_table, const float *d_taps, float kinv, int style, float fill,
                              hipStream_t st);
+hipError_t launch_dense_mfma_batch(int variant, const MfmaArgs *reqs, const int *tile_req, const int *tile_first,
+                                   int ntiles, hipStream_t st);
 
 // IIR: one cascade stage along canonical axis `axis`, FP64 recursion.
 struct IirCoef {
@@ -203,6 +228,18 @@ hipError_t launch_iir(const Dense4 &in, Dense4 &out, int axis, const IirCoef &c,
                       hipStream_t st);
 
 // Order filters.
+struct MedianArgs {
+  const void *in;
+  void *out;
+  int n0, n1;
+  int o0, o1;
+  int f0, f1;
+  int64_t in_plane, out_plane;
+  int nplanes, o2, n2, f2, f3;
+  int style;
+  int dt;
+  uint32_t fillkey;
+};
 hipError_t launch_median(const Dense4 &in, Dense4 &out, const int64_t *out_first, const int *wfirst,
                          const int *wlen, int style, double fill, hipStream_t st);
 // Separable min/max over truncated windows; with fill, the fill key joins any
@@ -217,5 +254,62 @@ size_t fft_workspace(const int64_t *N, int nd);
 hipError_t launch_fft_filter(const Dense4 &in, Dense4 &out, const int64_t *out_first, const DenseKernel &k,
                              const int64_t *Npad, int nd, const int64_t *lo, int style, float fill, void *work,
                              hipStream_t st);
+
+// Batched serving.  Each kernel takes an array of per-request descriptors in
+// device memory, the request of every tile (tile_req) and the first tile of
+// every request (tile_first); requests are 2-D images filtered over their
+// whole extent.
+struct SepReq {
+  const void *in;
+  void *out;
+  const float *taps0, *taps1;  // device FP32 taps
+  int n0, n1;
+  int k0, k1, first0, first1;
+  int style, dt_in, dt_out;
+  float fill;
+};
+hipError_t launch_sep_batch(const SepReq *reqs, const int *tile_req, const int *tile_first, int ntiles,
+                            size_t lds_bytes, hipStream_t st);
+void median_batch_tile(int k, bool pack, int *bx, int *by);
+uint32_t median_fill_key(int dt, double fill);
+hipError_t launch_median_batch(int k, bool pack, const MedianArgs *reqs, const int *tile_req, const int *tile_first,
+                               int ntiles, hipStream_t st);
+hipError_t launch_fill_uniform(void *data, int dtype, int64_t count, uint64_t seed, hipStream_t st);
+
+}  // namespace ifm
+
+// The context behind the opaque ifm_ctx of the C ABI.
+struct ifm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::vector<std::pair<void *, size_t>> slots;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // Device copies of dense kernels keyed by a hash of their FP32 taps, so a
+  // kernel used call after call is uploaded once (no per-call copy or sync).
+  struct TapEntry {
+    uint64_t hash;
+    size_t n;
+    void *dev;                // FP32 taps
+    void *table = nullptr;    // MFMA Toeplitz fragments (built on first MFMA use)
+    float kinv = 1.f;
+    int kx = 0, ky = 0;
+  };
+  std::vector<TapEntry> taps;
+  // Batched runs: a ring of descriptor staging slots (pinned host + device),
+  // each reusable once its event has completed.
+  struct BatchSlot {
+    void *host = nullptr;
+    void *dev = nullptr;
+    size_t cap = 0;
+    hipEvent_t ev = nullptr;
+    bool pending = false;
+  };
+  std::vector<BatchSlot> bslots;
+  int bnext = 0;
+};
+
+namespace ifm {
+void batch_release(ifm_ctx *ctx);
 
 }  // namespace ifm

@@ -4,7 +4,7 @@
 // kernels apply the border index map on load, so the reference's padded copy
 // (src/imfilter.jl:336) is never built: a factor cascade reads the image once
 // per fused group of axes.
-#include "ifm_internal.h"
+#include "k_tiles.h"
 
 namespace ifm {
 
@@ -176,10 +176,6 @@ hipError_t launch_fir1d(const Dense4 &in, const int64_t *in_first, Dense4 &out, 
 // LDS once (border map on load), runs the axis-0 factor into a second LDS
 // tile and the axis-1 factor from there, so HBM sees one read and one write.
 // ---------------------------------------------------------------------------
-constexpr int S2_TX = 64;
-constexpr int S2_TY = 16;
-constexpr int S2_MAXK = 64;
-
 struct Sep2dArgs {
   const void *in;
   void *out;
@@ -200,69 +196,73 @@ struct Sep2dArgs {
 
 __global__ void __launch_bounds__(256) k_sep2d(Sep2dArgs a) {
   extern __shared__ float smem[];
-  const int IN_W = S2_TX + a.k0 - 1;
-  const int IN_H = S2_TY + a.k1 - 1;
-  const int IN_WP = IN_W | 1;
-  float *s_in = smem;                     // IN_H x IN_WP
-  float *s_mid = smem + IN_H * IN_WP;     // IN_H x S2_TX
-  const int tid = threadIdx.x;
-  const int tx = tid & 63, ty = tid >> 6;
-  const int x0 = a.f0 + blockIdx.x * S2_TX;   // image coordinate of the tile's first output
-  const int y0 = a.f1 + blockIdx.y * S2_TY;
-  const void *in = a.in;
+  Sep2dTile t;
+  t.in = a.in;
+  t.out = a.out;
+  t.n0 = a.n0;
+  t.n1 = a.n1;
+  t.o0 = a.o0;
+  t.o1 = a.o1;
+  t.k0 = a.k0;
+  t.k1 = a.k1;
+  t.first0 = a.first0;
+  t.first1 = a.first1;
+  t.style = a.style;
+  t.dt_in = a.dt_in;
+  t.dt_out = a.dt_out;
+  t.fill = a.fill;
+  t.ox0 = blockIdx.x * S2_TX;
+  t.oy0 = blockIdx.y * S2_TY;
+  t.x0 = a.f0 + t.ox0;
+  t.y0 = a.f1 + t.oy0;
   for (int plane = blockIdx.z; plane < a.nplanes; plane += gridDim.z) {
-  const int p2 = plane % a.o2, p3 = plane / a.o2;
-  const int64_t ibase = ((int64_t)(p2 + a.f2) + (int64_t)a.n2 * (p3 + a.f3)) * a.in_plane;
+    const int p2 = plane % a.o2, p3 = plane / a.o2;
+    t.ibase = ((int64_t)(p2 + a.f2) + (int64_t)a.n2 * (p3 + a.f3)) * a.in_plane;
+    t.obase = (int64_t)plane * a.out_plane;
+    sep2d_tile(t, a.taps0, a.taps1, smem);
+    __syncthreads();  // the next plane reuses the LDS tiles
+  }
+}
 
-  // Load the input tile (border map on load).
-  int sx[2];
-#pragma unroll
-  for (int cc = 0; cc < 2; ++cc) {
-    int c = tx + 64 * cc;
-    sx[cc] = (c < IN_W) ? bmap(a.style, x0 + a.first0 + c, a.n0) : 0;
-  }
-  for (int r = ty; r < IN_H; r += 4) {
-    int sy = bmap(a.style, y0 + a.first1 + r, a.n1);
-#pragma unroll
-    for (int cc = 0; cc < 2; ++cc) {
-      int c = tx + 64 * cc;
-      if (c < IN_W) {
-        float v = (sy < 0 || sx[cc] < 0) ? a.fill : load_f(in, ibase + (int64_t)sy * a.n0 + sx[cc], a.dt_in);
-        s_in[r * IN_WP + c] = v;
-      }
-    }
-  }
-  __syncthreads();
-  // Axis-0 factor.
-  for (int r = ty; r < IN_H; r += 4) {
-    const float *row = s_in + r * IN_WP + tx;
-    float acc = 0.f;
-    for (int t = 0; t < a.k0; ++t) acc = fmaf(a.taps0[t], row[t], acc);
-    s_mid[r * S2_TX + tx] = acc;
-  }
-  __syncthreads();
-  // Axis-1 factor: each thread owns 4 consecutive rows of one column.
-  const int r0 = ty * 4;
-  float acc[4] = {0.f, 0.f, 0.f, 0.f};
-  for (int u = 0; u < a.k1 + 3; ++u) {
-    float v = s_mid[(r0 + u) * S2_TX + tx];
-#pragma unroll
-    for (int o = 0; o < 4; ++o) {
-      int j = u - o;
-      if (j >= 0 && j < a.k1) acc[o] = fmaf(a.taps1[j], v, acc[o]);
-    }
-  }
-  const int ox = blockIdx.x * S2_TX + tx;
-  if (ox < a.o0) {
-    const int64_t obase = (int64_t)plane * a.out_plane;
-#pragma unroll
-    for (int o = 0; o < 4; ++o) {
-      int oy = blockIdx.y * S2_TY + r0 + o;
-      if (oy < a.o1) store_f(a.out, obase + (int64_t)oy * a.o0 + ox, a.dt_out, acc[o]);
-    }
-  }
-  __syncthreads();  // the next plane reuses the LDS tiles
-  }
+// Batched serving: tile g of the launch is tile g - tile_first[r] of request
+// r = tile_req[g] (row-major over its 64 x 32 tiles).
+__global__ void __launch_bounds__(256) k_sep_batch(const SepReq *__restrict__ reqs, const int *__restrict__ tile_req,
+                                                   const int *__restrict__ tile_first) {
+  extern __shared__ float smem[];
+  const int g = blockIdx.x;
+  const int r = tile_req[g];
+  const SepReq &q = reqs[r];
+  const int local = g - tile_first[r];
+  const int ntx = (q.n0 + S2_TX - 1) / S2_TX;
+  Sep2dTile t;
+  t.in = q.in;
+  t.out = q.out;
+  t.ibase = 0;
+  t.obase = 0;
+  t.n0 = q.n0;
+  t.n1 = q.n1;
+  t.o0 = q.n0;
+  t.o1 = q.n1;
+  t.ox0 = (local % ntx) * S2_TX;
+  t.oy0 = (local / ntx) * S2_TY;
+  t.x0 = t.ox0;
+  t.y0 = t.oy0;
+  t.k0 = q.k0;
+  t.k1 = q.k1;
+  t.first0 = q.first0;
+  t.first1 = q.first1;
+  t.style = q.style;
+  t.dt_in = q.dt_in;
+  t.dt_out = q.dt_out;
+  t.fill = q.fill;
+  sep2d_tile(t, q.taps0, q.taps1, smem);
+}
+
+hipError_t launch_sep_batch(const SepReq *reqs, const int *tile_req, const int *tile_first, int ntiles,
+                            size_t lds_bytes, hipStream_t st) {
+  if (ntiles == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sep_batch, dim3(ntiles), dim3(256), lds_bytes, st, reqs, tile_req, tile_first);
+  return hipGetLastError();
 }
 
 hipError_t launch_sep2d(const Dense4 &in, Dense4 &out, const int64_t *out_first, const FirPass &p0,
@@ -294,8 +294,7 @@ hipError_t launch_sep2d(const Dense4 &in, Dense4 &out, const int64_t *out_first,
   for (int t = 0; t < p0.len; ++t) a.taps0[t] = p0.taps[t];
   for (int t = 0; t < p1.len; ++t) a.taps1[t] = p1.taps[t];
   if (out.count() == 0) return hipSuccess;
-  const int IN_W = S2_TX + a.k0 - 1, IN_H = S2_TY + a.k1 - 1;
-  size_t lds = sizeof(float) * ((size_t)IN_H * (IN_W | 1) + (size_t)IN_H * S2_TX);
+  const size_t lds = sizeof(float) * sep2d_lds_floats(a.k0, a.k1);
   dim3 grid((a.o0 + S2_TX - 1) / S2_TX, (a.o1 + S2_TY - 1) / S2_TY, a.nplanes < 65535 ? a.nplanes : 65535);
   hipLaunchKernelGGL(k_sep2d, grid, dim3(256), lds, st, a);
   return hipGetLastError();

@@ -56,25 +56,6 @@ constexpr int MF_UCH = 8;             // kernel rows per chunk
 constexpr int MF_MAXCH = 4;           // kernel rows up to 32
 constexpr int MF_MAXKB = 2;           // kernel columns up to 49
 
-struct MfmaArgs {
-  const void *in;
-  void *out;
-  const uint16_t *ttab;  // [chunk][u][hi/lo][lane][8] FP16 Toeplitz fragments
-  const float *taps;     // Kx * Ky FP32 taps (direct path of non-finite tiles)
-  float kinv;            // 1 / kernel scale
-  int n0, n1;            // input plane
-  int o0, o1;            // output plane
-  int f0, f1;            // out_first along axes 0 and 1
-  int kx, ky;
-  int first0, first1;    // kernel first offsets
-  int ntx, nty, ntiles;  // tiles along x, along y, in total over all planes
-  int64_t in_plane, out_plane;
-  int o2, n2, f2, f3;    // plane mapping for axes 2 and 3
-  int style;
-  int dt_in, dt_out;
-  float fill;
-};
-
 // LDS layout of a tile.  A fragment read has lane (yy, g) read 8 words of
 // tile row R0 + 8*yy, so rows are grouped by R mod 8: tile row R sits in LDS
 // row (R % 8) * RB + R / 8, and the 16 rows one read touches are consecutive
@@ -436,8 +417,31 @@ __device__ __forceinline__ float wave_max(float v) {
   return v;
 }
 
-template <int KB, int NCH>
-__global__ void __launch_bounds__(256, 2) k_dense_mfma(MfmaArgs a) {
+// Arguments of global tile t: the launch's own (one image), or those of
+// request tile_req[t] (batched serving); local = the tile's index within its
+// image, r = its request.
+struct MfBatch {
+  const MfmaArgs *reqs;
+  const int *tile_req;
+  const int *tile_first;
+};
+
+template <bool BATCH>
+__device__ __forceinline__ const MfmaArgs &mf_args(const MfmaArgs &a0, const MfBatch &bt, int t, int &local,
+                                                   int &r) {
+  if constexpr (BATCH) {
+    r = bt.tile_req[t];
+    local = t - bt.tile_first[r];
+    return bt.reqs[r];
+  } else {
+    r = 0;
+    local = t;
+    return a0;
+  }
+}
+
+template <int KB, int NCH, bool BATCH>
+__global__ void __launch_bounds__(256, 2) k_dense_mfma(MfmaArgs a0, MfBatch bt) {
   using G = MfmaGeom<KB, NCH>;
   constexpr int NC = G::NC;
   // One 32-bit word per tile element: the packed FP16 pair hi | lo << 16
@@ -450,13 +454,15 @@ __global__ void __launch_bounds__(256, 2) k_dense_mfma(MfmaArgs a) {
   const int lrow = lane & 15, lgrp = lane >> 4;
 
   // A contiguous run of tiles per workgroup; run lengths differ by at most one.
-  const int t0 = (int)((int64_t)blockIdx.x * a.ntiles / gridDim.x);
-  const int t1 = (int)((int64_t)(blockIdx.x + 1) * a.ntiles / gridDim.x);
+  const int t0 = (int)((int64_t)blockIdx.x * a0.ntiles / gridDim.x);
+  const int t1 = (int)((int64_t)(blockIdx.x + 1) * a0.ntiles / gridDim.x);
   if (t0 >= t1) return;  // the whole workgroup leaves together
 
+  int lf, cur;
+  const MfmaArgs &af = mf_args<BATCH>(a0, bt, t0, lf, cur);
   f16x8 ahi[MF_UCH], alo[MF_UCH], bhi[MF_UCH], blo[MF_UCH];
-  mf_tload(a.ttab, 0, ahi, alo);
-  if constexpr (NC > 1) mf_tload(a.ttab, 1, bhi, blo);
+  mf_tload(af.ttab, 0, ahi, alo);
+  if constexpr (NC > 1) mf_tload(af.ttab, 1, bhi, blo);
 
   // Border maps: with one staging batch they are written between the two
   // barriers of the previous tile and read by the loads issued after the
@@ -464,16 +470,18 @@ __global__ void __launch_bounds__(256, 2) k_dense_mfma(MfmaArgs a) {
   __shared__ int rmap[G::RMAP], cmap[G::TWP];
   uint32_t raw[G::JB];
   if constexpr (G::NB == 1) {
-    const MfTile T0 = mf_tile(a, t0);
-    mf_maps<G>(a, T0, rmap, cmap);
+    const MfTile T0 = mf_tile(af, lf);
+    mf_maps<G>(af, T0, rmap, cmap);
     __syncthreads();
-    mf_issue_dt<G, 0>(a, T0, rmap, cmap, raw);
+    mf_issue_dt<G, 0>(af, T0, rmap, cmap, raw);
   }
 
   const uint32_t *lw = lds32 + lrow * G::P + 16 * wave + 8 * lgrp;
   int par = 0;
   for (int t = t0; t < t1; ++t, par ^= 1) {
-    const MfTile T = mf_tile(a, t);
+    int lt, rt;
+    const MfmaArgs &a = mf_args<BATCH>(a0, bt, t, lt, rt);
+    const MfTile T = mf_tile(a, lt);
     float amax = 0.f;
     bool bad = false;
     if constexpr (G::NB == 1) {
@@ -504,10 +512,12 @@ __global__ void __launch_bounds__(256, 2) k_dense_mfma(MfmaArgs a) {
       mf_direct(a, T);
       if constexpr (G::NB == 1) {
         if (t + 1 < t1) {
-          const MfTile Tn = mf_tile(a, t + 1);
-          mf_maps<G>(a, Tn, rmap, cmap);
+          int ln, rn;
+          const MfmaArgs &an = mf_args<BATCH>(a0, bt, t + 1, ln, rn);
+          const MfTile Tn = mf_tile(an, ln);
+          mf_maps<G>(an, Tn, rmap, cmap);
           __syncthreads();
-          mf_issue_dt<G, 0>(a, Tn, rmap, cmap, raw);
+          mf_issue_dt<G, 0>(an, Tn, rmap, cmap, raw);
         }
       }
       continue;
@@ -520,13 +530,29 @@ __global__ void __launch_bounds__(256, 2) k_dense_mfma(MfmaArgs a) {
     const float xinv = ldexpf(1.0f, e);
     if constexpr (G::NB == 1) {
       mf_put<G, 0>(lds32, raw, xs, true);
-      if (t + 1 < t1) mf_maps<G>(a, mf_tile(a, t + 1), rmap, cmap);
+      if (t + 1 < t1) {
+        int ln, rn;
+        const MfmaArgs &an = mf_args<BATCH>(a0, bt, t + 1, ln, rn);
+        mf_maps<G>(an, mf_tile(an, ln), rmap, cmap);
+      }
     } else {
       mf_convert<G>(lds32, xs);
     }
     __syncthreads();
-    if constexpr (G::NB == 1)
-      if (t + 1 < t1) mf_issue_dt<G, 0>(a, mf_tile(a, t + 1), rmap, cmap, raw);
+    if constexpr (G::NB == 1) {
+      if (t + 1 < t1) {
+        int ln, rn;
+        const MfmaArgs &an = mf_args<BATCH>(a0, bt, t + 1, ln, rn);
+        mf_issue_dt<G, 0>(an, mf_tile(an, ln), rmap, cmap, raw);
+      }
+    }
+    if constexpr (BATCH) {
+      if (rt != cur) {  // another request: its Toeplitz fragments
+        mf_tload(a.ttab, 0, ahi, alo);
+        if constexpr (NC > 1) mf_tload(a.ttab, 1, bhi, blo);
+        cur = rt;
+      }
+    }
 
     f32x4 acc[MF_B];
 #pragma unroll
@@ -563,15 +589,17 @@ static int mfma_nch(int ky) {
   return p;
 }
 
-typedef void (*MfKernel)(MfmaArgs);
+typedef void (*MfKernel)(MfmaArgs, MfBatch);
 struct MfVariant {
-  MfKernel fn;
+  MfKernel fn;   // one image
+  MfKernel fnb;  // batched requests
   size_t lds;
 };
 
 template <int KB, int NCH>
 static MfVariant mf_variant() {
-  return {k_dense_mfma<KB, NCH>, sizeof(uint32_t) * (size_t)MfmaGeom<KB, NCH>::LDS_WORDS};
+  return {k_dense_mfma<KB, NCH, false>, k_dense_mfma<KB, NCH, true>,
+          sizeof(uint32_t) * (size_t)MfmaGeom<KB, NCH>::LDS_WORDS};
 }
 
 static bool mf_select(int kb, int nch, MfVariant *v) {
@@ -633,13 +661,13 @@ void dense_mfma_table(const DenseKernel &k, std::vector<uint16_t> &tab, float *k
         }
 }
 
-hipError_t launch_dense_mfma(const Dense4 &in, Dense4 &out, const int64_t *out_first, const DenseKernel &k,
-                             const uint16_t *d_table, const float *d_taps, float kinv, int style, float fill,
-                             hipStream_t st) {
-  if (out.count() == 0) return hipSuccess;
+int dense_mfma_variant(const DenseKernel &k) { return mfma_kb(k.n[0]) * 16 + mfma_nch(k.n[1]); }
+
+bool dense_mfma_args(const Dense4 &in, Dense4 &out, const int64_t *out_first, const DenseKernel &k,
+                     const uint16_t *d_table, const float *d_taps, float kinv, int style, float fill, MfmaArgs *ap) {
   MfVariant v;
-  if (!mf_select(mfma_kb(k.n[0]), mfma_nch(k.n[1]), &v)) return hipErrorInvalidValue;
-  MfmaArgs a;
+  if (!mf_select(mfma_kb(k.n[0]), mfma_nch(k.n[1]), &v)) return false;
+  MfmaArgs &a = *ap;
   a.in = in.data;
   a.out = out.data;
   a.ttab = d_table;
@@ -668,13 +696,41 @@ hipError_t launch_dense_mfma(const Dense4 &in, Dense4 &out, const int64_t *out_f
   a.ntx = (a.o0 + MF_TX - 1) / MF_TX;
   a.nty = (a.o1 + MF_TY - 1) / MF_TY;
   const int64_t ntiles = (int64_t)a.ntx * a.nty * out.n[2] * out.n[3];
-  if (ntiles >= (int64_t)1 << 31) return hipErrorInvalidValue;
+  if (ntiles >= (int64_t)1 << 31) return false;
   a.ntiles = (int)ntiles;
+  return true;
+}
+
+static int64_t mf_grid(int64_t ntiles) {
   static int cus = -1;
   if (cus < 0) cus = device_attr(hipDeviceAttributeMultiprocessorCount, 256);
-  const int64_t grid = ntiles < 2 * (int64_t)cus ? ntiles : 2 * (int64_t)cus;
-  void *args[] = {&a};
-  return hipLaunchKernel((const void *)v.fn, dim3((unsigned)grid), dim3(256), args, v.lds, st);
+  return ntiles < 2 * (int64_t)cus ? ntiles : 2 * (int64_t)cus;
+}
+
+hipError_t launch_dense_mfma(const Dense4 &in, Dense4 &out, const int64_t *out_first, const DenseKernel &k,
+                             const uint16_t *d_table, const float *d_taps, float kinv, int style, float fill,
+                             hipStream_t st) {
+  if (out.count() == 0) return hipSuccess;
+  MfVariant v;
+  MfmaArgs a;
+  if (!mf_select(mfma_kb(k.n[0]), mfma_nch(k.n[1]), &v) ||
+      !dense_mfma_args(in, out, out_first, k, d_table, d_taps, kinv, style, fill, &a))
+    return hipErrorInvalidValue;
+  MfBatch none = {nullptr, nullptr, nullptr};
+  void *args[] = {&a, &none};
+  return hipLaunchKernel((const void *)v.fn, dim3((unsigned)mf_grid(a.ntiles)), dim3(256), args, v.lds, st);
+}
+
+hipError_t launch_dense_mfma_batch(int variant, const MfmaArgs *reqs, const int *tile_req, const int *tile_first,
+                                   int ntiles, hipStream_t st) {
+  if (ntiles == 0) return hipSuccess;
+  MfVariant v;
+  if (!mf_select(variant / 16, variant % 16, &v)) return hipErrorInvalidValue;
+  MfmaArgs a = {};
+  a.ntiles = ntiles;  // the launch's tile count; everything else comes from reqs
+  MfBatch bt = {reqs, tile_req, tile_first};
+  void *args[] = {&a, &bt};
+  return hipLaunchKernel((const void *)v.fnb, dim3((unsigned)mf_grid(ntiles)), dim3(256), args, v.lds, st);
 }
 
 }  // namespace ifm

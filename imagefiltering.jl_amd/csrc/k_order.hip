@@ -35,90 +35,108 @@ template <> struct MedianNet<7, 2, 2> {
   template <typename T> static __device__ __forceinline__ void run(const T (&v)[64], T (&o)[4]) { median_net_7_2_2(v, o); }
 };
 
-struct MedianArgs {
-  const void *in;
-  void *out;
-  int n0, n1;
-  int o0, o1;
-  int f0, f1;
-  int64_t in_plane, out_plane;
-  int nplanes, o2, n2, f2, f3;
-  int style;
-  int dt;
-  uint32_t fillkey;
-};
 
 // Block = 16 x 16 threads; thread tile TX x TY outputs, PACK tiles stacked in
 // rows (PACK = 2 packs two 16-bit keys per register).
 template <int K, int TX, int TY, int PACK>
-__global__ void __launch_bounds__(256) k_median_net(MedianArgs a) {
-  constexpr int H = K / 2;
-  constexpr int BX = 16 * TX;
-  constexpr int BYH = 16 * TY;         // rows per packed half
-  constexpr int BY = BYH * PACK;
-  constexpr int LW = BX + K - 1;
-  constexpr int LH = BY + K - 1;
-  constexpr int VW = TX + K - 1, VH = TY + K - 1;
-  __shared__ uint32_t tile[LH * LW];
+struct NetGeom {
+  static constexpr int H = K / 2;
+  static constexpr int BX = 16 * TX;
+  static constexpr int BYH = 16 * TY;  // rows per packed half
+  static constexpr int BY = BYH * PACK;
+  static constexpr int LW = BX + K - 1;
+  static constexpr int LH = BY + K - 1;
+  static constexpr int VW = TX + K - 1, VH = TY + K - 1;
+};
+
+// One BX x BY output tile whose first output is (bx, by) in output indices,
+// image coordinate (a.f0 + bx, a.f1 + by), of the plane at ibase / obase.
+template <int K, int TX, int TY, int PACK>
+__device__ __forceinline__ void median_tile(const MedianArgs &a, int bx, int by, int64_t ibase, int64_t obase,
+                                            uint32_t *tile) {
+  using G = NetGeom<K, TX, TY, PACK>;
+  constexpr int LW = G::LW, LH = G::LH, VW = G::VW, VH = G::VH, BYH = G::BYH;
   const int tid = threadIdx.x;
   const int tx = tid & 15, ty = tid >> 4;
-  const int x0 = a.f0 + blockIdx.x * BX;   // image coordinate of first output column
-  const int y0 = a.f1 + blockIdx.y * BY;
+  const int x0 = a.f0 + bx, y0 = a.f1 + by;
+  for (int i = tid; i < LH * LW; i += 256) {
+    int r = i / LW, c = i - r * LW;
+    int sy = bmap(a.style, y0 - G::H + r, a.n1);
+    int sx = bmap(a.style, x0 - G::H + c, a.n0);
+    tile[i] = (sy < 0 || sx < 0) ? a.fillkey : key32(a.in, ibase + (int64_t)sy * a.n0 + sx, a.dt);
+  }
+  __syncthreads();
+  const int cx = tx * TX, cy = ty * TY;
+  if constexpr (PACK == 2) {
+    u16x2 v[VW * VH];
+#pragma unroll
+    for (int r = 0; r < VH; ++r)
+#pragma unroll
+      for (int c = 0; c < VW; ++c) {
+        u16x2 p;
+        p.x = (uint16_t)tile[(cy + r) * LW + cx + c];
+        p.y = (uint16_t)tile[(cy + BYH + r) * LW + cx + c];
+        v[c + VW * r] = p;
+      }
+    u16x2 o[TX * TY];
+    MedianNet<K, TX, TY>::run(v, o);
+#pragma unroll
+    for (int r = 0; r < TY; ++r)
+#pragma unroll
+      for (int c = 0; c < TX; ++c) {
+        int ox = bx + cx + c;
+        int oy = by + cy + r;
+        if (ox < a.o0) {
+          if (oy < a.o1) store_key32(a.out, obase + (int64_t)oy * a.o0 + ox, a.dt, o[c + TX * r].x);
+          if (oy + BYH < a.o1) store_key32(a.out, obase + (int64_t)(oy + BYH) * a.o0 + ox, a.dt, o[c + TX * r].y);
+        }
+      }
+  } else {
+    uint32_t v[VW * VH];
+#pragma unroll
+    for (int r = 0; r < VH; ++r)
+#pragma unroll
+      for (int c = 0; c < VW; ++c) v[c + VW * r] = tile[(cy + r) * LW + cx + c];
+    uint32_t o[TX * TY];
+    MedianNet<K, TX, TY>::run(v, o);
+#pragma unroll
+    for (int r = 0; r < TY; ++r)
+#pragma unroll
+      for (int c = 0; c < TX; ++c) {
+        int ox = bx + cx + c;
+        int oy = by + cy + r;
+        if (ox < a.o0 && oy < a.o1) store_key32(a.out, obase + (int64_t)oy * a.o0 + ox, a.dt, o[c + TX * r]);
+      }
+  }
+}
+
+template <int K, int TX, int TY, int PACK>
+__global__ void __launch_bounds__(256) k_median_net(MedianArgs a) {
+  using G = NetGeom<K, TX, TY, PACK>;
+  __shared__ uint32_t tile[G::LH * G::LW];
   for (int plane = blockIdx.z; plane < a.nplanes; plane += gridDim.z) {
     const int p2 = plane % a.o2, p3 = plane / a.o2;
     const int64_t ibase = ((int64_t)(p2 + a.f2) + (int64_t)a.n2 * (p3 + a.f3)) * a.in_plane;
-    for (int i = tid; i < LH * LW; i += 256) {
-      int r = i / LW, c = i - r * LW;
-      int sy = bmap(a.style, y0 - H + r, a.n1);
-      int sx = bmap(a.style, x0 - H + c, a.n0);
-      tile[i] = (sy < 0 || sx < 0) ? a.fillkey : key32(a.in, ibase + (int64_t)sy * a.n0 + sx, a.dt);
-    }
-    __syncthreads();
-    const int64_t obase = (int64_t)plane * a.out_plane;
-    const int cx = tx * TX, cy = ty * TY;
-    if constexpr (PACK == 2) {
-      u16x2 v[VW * VH];
-#pragma unroll
-      for (int r = 0; r < VH; ++r)
-#pragma unroll
-        for (int c = 0; c < VW; ++c) {
-          u16x2 p;
-          p.x = (uint16_t)tile[(cy + r) * LW + cx + c];
-          p.y = (uint16_t)tile[(cy + BYH + r) * LW + cx + c];
-          v[c + VW * r] = p;
-        }
-      u16x2 o[TX * TY];
-      MedianNet<K, TX, TY>::run(v, o);
-#pragma unroll
-      for (int r = 0; r < TY; ++r)
-#pragma unroll
-        for (int c = 0; c < TX; ++c) {
-          int ox = blockIdx.x * BX + cx + c;
-          int oy = blockIdx.y * BY + cy + r;
-          if (ox < a.o0) {
-            if (oy < a.o1) store_key32(a.out, obase + (int64_t)oy * a.o0 + ox, a.dt, o[c + TX * r].x);
-            if (oy + BYH < a.o1) store_key32(a.out, obase + (int64_t)(oy + BYH) * a.o0 + ox, a.dt, o[c + TX * r].y);
-          }
-        }
-    } else {
-      uint32_t v[VW * VH];
-#pragma unroll
-      for (int r = 0; r < VH; ++r)
-#pragma unroll
-        for (int c = 0; c < VW; ++c) v[c + VW * r] = tile[(cy + r) * LW + cx + c];
-      uint32_t o[TX * TY];
-      MedianNet<K, TX, TY>::run(v, o);
-#pragma unroll
-      for (int r = 0; r < TY; ++r)
-#pragma unroll
-        for (int c = 0; c < TX; ++c) {
-          int ox = blockIdx.x * BX + cx + c;
-          int oy = blockIdx.y * BY + cy + r;
-          if (ox < a.o0 && oy < a.o1) store_key32(a.out, obase + (int64_t)oy * a.o0 + ox, a.dt, o[c + TX * r]);
-        }
-    }
+    median_tile<K, TX, TY, PACK>(a, blockIdx.x * G::BX, blockIdx.y * G::BY, ibase, (int64_t)plane * a.out_plane,
+                                 tile);
     __syncthreads();
   }
+}
+
+// Batched serving: tile g belongs to request tile_req[g]; its requests are
+// 2-D images whose output covers the whole image.
+template <int K, int TX, int TY, int PACK>
+__global__ void __launch_bounds__(256) k_median_batch(const MedianArgs *__restrict__ reqs,
+                                                      const int *__restrict__ tile_req,
+                                                      const int *__restrict__ tile_first) {
+  using G = NetGeom<K, TX, TY, PACK>;
+  __shared__ uint32_t tile[G::LH * G::LW];
+  const int g = blockIdx.x;
+  const int r = tile_req[g];
+  const MedianArgs &a = reqs[r];
+  const int local = g - tile_first[r];
+  const int ntx = (a.o0 + G::BX - 1) / G::BX;
+  median_tile<K, TX, TY, PACK>(a, (local % ntx) * G::BX, (local / ntx) * G::BY, 0, 0, tile);
 }
 
 // Generic window median: the thread enumerates its window twice (candidate,
@@ -251,6 +269,39 @@ static void launch_net_on(const MedianArgs &a, hipStream_t st) {
   dim3 grid((a.o0 + BX - 1) / BX, (a.o1 + BY - 1) / BY, a.nplanes < 65535 ? a.nplanes : 65535);
   hipLaunchKernelGGL((k_median_net<K, TX, TY, PACK>), grid, dim3(256), 0, st, a);
 }
+
+// Batched median: tile geometry of the network kernel for window k.
+void median_batch_tile(int k, bool pack, int *bx, int *by) {
+  const int tx = k == 3 ? 4 : 2, ty = 2;
+  *bx = 16 * tx;
+  *by = 16 * ty * (pack ? 2 : 1);
+}
+
+template <int K, int TX, int TY, int PACK>
+static void launch_batch_on(const MedianArgs *reqs, const int *tile_req, const int *tile_first, int ntiles,
+                            hipStream_t st) {
+  hipLaunchKernelGGL((k_median_batch<K, TX, TY, PACK>), dim3(ntiles), dim3(256), 0, st, reqs, tile_req, tile_first);
+}
+
+hipError_t launch_median_batch(int k, bool pack, const MedianArgs *reqs, const int *tile_req, const int *tile_first,
+                               int ntiles, hipStream_t st) {
+  if (ntiles == 0) return hipSuccess;
+  if (k == 3) {
+    if (pack) launch_batch_on<3, 4, 2, 2>(reqs, tile_req, tile_first, ntiles, st);
+    else launch_batch_on<3, 4, 2, 1>(reqs, tile_req, tile_first, ntiles, st);
+  } else if (k == 5) {
+    if (pack) launch_batch_on<5, 2, 2, 2>(reqs, tile_req, tile_first, ntiles, st);
+    else launch_batch_on<5, 2, 2, 1>(reqs, tile_req, tile_first, ntiles, st);
+  } else {
+    if (pack) launch_batch_on<7, 2, 2, 2>(reqs, tile_req, tile_first, ntiles, st);
+    else launch_batch_on<7, 2, 2, 1>(reqs, tile_req, tile_first, ntiles, st);
+  }
+  return hipGetLastError();
+}
+
+// The fill key of a median request (the fill value converted to the input
+// eltype, as an order key).
+uint32_t median_fill_key(int dt, double fill) { return fill_key32(dt, fill); }
 
 hipError_t launch_median(const Dense4 &in, Dense4 &out, const int64_t *out_first, const int *wfirst,
                          const int *wlen, int style, double fill, hipStream_t st) {

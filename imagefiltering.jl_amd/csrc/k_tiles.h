@@ -1,0 +1,142 @@
+// k_tiles.h — tile bodies shared by the single-image kernels and the batched
+// serving kernels (gfx950).
+#pragma once
+#include "ifm_internal.h"
+
+namespace ifm {
+
+// ---------------------------------------------------------------------------
+// Fused pair of 1-D factors on axes 0 and 1 (the two-factor cascade of
+// src/imfilter.jl:372-457 with the factor loop of :671-739).  One workgroup of
+// 256 threads computes a 64 x 32 output tile:
+//   1. the (32 + k1 - 1) x (64 + k0 - 1) input tile is loaded once into LDS,
+//      border map on load; each thread owns one column and every other row,
+//      and issues 8 rows of loads before it writes any of them;
+//   2. the axis-0 factor runs from registers: a thread slides over k0 + 3
+//      LDS values to produce 4 consecutive outputs of one row (mid tile);
+//   3. the axis-1 factor likewise produces 8 consecutive rows of one column.
+// Accumulation is FP32 in tap order, like the reference's loops.
+// ---------------------------------------------------------------------------
+constexpr int S2_TX = 64;
+constexpr int S2_TY = 32;
+constexpr int S2_MAXK = 64;
+
+struct Sep2dTile {
+  const void *in;
+  void *out;
+  int64_t ibase, obase;  // plane offsets (elements)
+  int n0, n1;            // input extents
+  int o0, o1;            // output extents
+  int x0, y0;            // input coordinate of the tile's first output
+  int ox0, oy0;          // output index of the tile's first output
+  int k0, k1, first0, first1;
+  int style, dt_in, dt_out;
+  float fill;
+};
+
+// LDS: input tile (odd pitch), then the mid tile at a 16-byte boundary.
+__host__ __device__ inline int sep2d_mid_offset(int k0, int k1) {
+  const int in_h = S2_TY + k1 - 1, in_w = S2_TX + k0 - 1;
+  return (in_h * (in_w | 1) + 3) & ~3;
+}
+__host__ __device__ inline size_t sep2d_lds_floats(int k0, int k1) {
+  return (size_t)sep2d_mid_offset(k0, k1) + (size_t)(S2_TY + k1 - 1) * S2_TX;
+}
+
+template <int DT>
+__device__ __forceinline__ float load_as(const void *p, int64_t i) {
+  if constexpr (DT == IFM_U8) return (float)((const uint8_t *)p)[i];
+  else if constexpr (DT == IFM_N0F8) return (float)((const uint8_t *)p)[i] * (1.0f / 255.0f);
+  else if constexpr (DT == IFM_F32) return ((const float *)p)[i];
+  else if constexpr (DT == IFM_F64) return (float)((const double *)p)[i];
+  else if constexpr (DT == IFM_BF16) return bf16_to_f32(((const uint16_t *)p)[i]);
+  else return __half2float(((const __half *)p)[i]);
+}
+
+template <int DT>
+__device__ __forceinline__ void sep2d_stage(const Sep2dTile &a, float *s_in, int in_w, int in_h, int in_wp) {
+  const int tid = threadIdx.x;
+  const int c = tid & 127;                               // the thread's column
+  const int rh = __builtin_amdgcn_readfirstlane(tid >> 7);  // rows rh, rh + 2, ...
+  const bool colok = c < in_w;
+  const int sx = bmap(a.style, a.x0 + a.first0 + (colok ? c : in_w - 1), a.n0);
+  const int nrow = (in_h - rh + 1) >> 1;                // rows of this half
+  for (int j0 = 0; j0 < nrow; j0 += 8) {
+    float v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int jj = j0 + j < nrow ? j0 + j : nrow - 1;  // clamped: every load is in bounds
+      const int sy = bmap(a.style, a.y0 + a.first1 + rh + 2 * jj, a.n1);
+      const bool fill = sy < 0 || sx < 0;
+      const float x = load_as<DT>(a.in, a.ibase + (fill ? 0 : (int64_t)sy * a.n0 + sx));
+      v[j] = fill ? a.fill : x;
+    }
+    if (colok) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (j0 + j < nrow) s_in[(rh + 2 * (j0 + j)) * in_wp + c] = v[j];
+    }
+  }
+}
+
+__device__ __forceinline__ void sep2d_tile(const Sep2dTile &a, const float *taps0, const float *taps1,
+                                           float *smem) {
+  const int in_w = S2_TX + a.k0 - 1;
+  const int in_h = S2_TY + a.k1 - 1;
+  const int in_wp = in_w | 1;
+  float *s_in = smem;                                     // in_h x in_wp
+  float *s_mid = smem + sep2d_mid_offset(a.k0, a.k1);    // in_h x 64
+  switch (a.dt_in) {
+  case IFM_F32: sep2d_stage<IFM_F32>(a, s_in, in_w, in_h, in_wp); break;
+  case IFM_BF16: sep2d_stage<IFM_BF16>(a, s_in, in_w, in_h, in_wp); break;
+  case IFM_F16: sep2d_stage<IFM_F16>(a, s_in, in_w, in_h, in_wp); break;
+  case IFM_F64: sep2d_stage<IFM_F64>(a, s_in, in_w, in_h, in_wp); break;
+  case IFM_N0F8: sep2d_stage<IFM_N0F8>(a, s_in, in_w, in_h, in_wp); break;
+  default: sep2d_stage<IFM_U8>(a, s_in, in_w, in_h, in_wp); break;
+  }
+  __syncthreads();
+  const int tid = threadIdx.x;
+  // Axis-0 factor: item = (row, quad of 4 columns).
+  for (int item = tid; item < in_h * 16; item += 256) {
+    const int row = item >> 4, cq = (item & 15) * 4;
+    const float *src = s_in + row * in_wp + cq;
+    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    float w[4] = {src[0], src[1], src[2], 0.f};
+    for (int t = 0; t < a.k0; ++t) {
+      w[3] = src[t + 3];
+      const float k = taps0[t];
+#pragma unroll
+      for (int o = 0; o < 4; ++o) acc[o] = fmaf(k, w[o], acc[o]);
+      w[0] = w[1];
+      w[1] = w[2];
+      w[2] = w[3];
+    }
+    float4 r = make_float4(acc[0], acc[1], acc[2], acc[3]);
+    *(float4 *)(s_mid + row * S2_TX + cq) = r;
+  }
+  __syncthreads();
+  // Axis-1 factor: each thread owns 8 consecutive rows of one column.
+  const int col = tid & 63, r0 = (tid >> 6) * 8;
+  float acc[8];
+#pragma unroll
+  for (int o = 0; o < 8; ++o) acc[o] = 0.f;
+  const float *src = s_mid + r0 * S2_TX + col;
+  for (int u = 0; u < a.k1 + 7; ++u) {
+    const float v = src[u * S2_TX];
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+      const int j = u - o;
+      if (j >= 0 && j < a.k1) acc[o] = fmaf(taps1[j], v, acc[o]);
+    }
+  }
+  const int ox = a.ox0 + col;
+  if (ox < a.o0) {
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+      const int oy = a.oy0 + r0 + o;
+      if (oy < a.o1) store_f(a.out, a.obase + (int64_t)oy * a.o0 + ox, a.dt_out, acc[o]);
+    }
+  }
+}
+
+}  // namespace ifm

@@ -21,7 +21,7 @@ U8, N0F8, F32, F64, BF16, F16 = range(6)
 # ifm_style
 REPLICATE, CIRCULAR, SYMMETRIC, REFLECT, FILL = range(5)
 # ifm_precision
-PREC_AUTO, PREC_FP32, PREC_BF16X3 = range(3)
+PREC_AUTO, PREC_FP32, PREC_F16X3 = range(3)
 
 i64x4 = C.c_int64 * MAXD
 
@@ -51,10 +51,27 @@ class IfmIir(C.Structure):
                 ("M", C.c_double * 9), ("axis", C.c_int32), ("reserved", C.c_int32)]
 
 
+# ifm_op
+OP_SEPARABLE, OP_DENSE, OP_MEDIAN = range(3)
+
+
+class IfmRequest(C.Structure):
+    _fields_ = [("op", C.c_int32), ("nfactors", C.c_int32), ("in_", IfmArray), ("out", IfmArray),
+                ("border", IfmBorder), ("factors", IfmFactor * 2), ("kernel", IfmKernel), ("window", C.c_int64),
+                ("nfactors2", C.c_int32), ("reserved", C.c_int32), ("out2", IfmArray),
+                ("factors2", IfmFactor * 2)]
+
+
+class IfmServeConfig(C.Structure):
+    _fields_ = [("budget_bytes", C.c_uint64), ("mode", C.c_int32), ("max_batch", C.c_int32),
+                ("max_inflight", C.c_int32), ("reserved", C.c_int32)]
+
+
 EXPORTS = [
     "ifm_ctx_create", "ifm_ctx_destroy", "ifm_ctx_set_stream", "ifm_ctx_synchronize", "ifm_last_error",
     "ifm_version", "ifm_padarray", "ifm_imfilter_fir", "ifm_imfilter_dense", "ifm_imfilter_fft",
     "ifm_imfilter_iir", "ifm_mapwindow_median", "ifm_mapwindow_minmax", "ifm_extrema_filter",
+    "ifm_batch_run", "ifm_serve", "ifm_fill_uniform",
     "ifm_device_alloc", "ifm_device_free", "ifm_memcpy_h2d", "ifm_memcpy_d2h", "ifm_memset_d",
     "ifm_timer_start", "ifm_timer_stop", "ifm_device_count",
 ]
@@ -93,6 +110,10 @@ def load(path=LIBPATH):
     L.ifm_mapwindow_minmax.argtypes = [vp, P(IfmArray), P(IfmArray), P(IfmArray), P(C.c_int64), P(C.c_int64),
                                        P(C.c_int64), P(IfmBorder)]
     L.ifm_extrema_filter.argtypes = [vp, P(IfmArray), P(IfmArray), P(IfmArray), P(C.c_int64)]
+    L.ifm_batch_run.argtypes = [vp, P(IfmRequest), C.c_int]
+    L.ifm_serve.argtypes = [vp, P(IfmRequest), P(C.c_double), C.c_int, P(IfmServeConfig), P(C.c_double),
+                            P(C.c_double)]
+    L.ifm_fill_uniform.argtypes = [vp, P(IfmArray), C.c_uint64]
     L.ifm_device_alloc.argtypes = [vp, C.c_uint64, P(vp)]
     L.ifm_device_free.argtypes = [vp, vp]
     L.ifm_memcpy_h2d.argtypes = [vp, vp, vp, C.c_uint64]

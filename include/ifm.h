@@ -56,9 +56,10 @@ typedef enum {
 
 /* Dense-kernel arithmetic for ifm_imfilter_dense. */
 typedef enum {
-  IFM_PREC_AUTO = 0,    /* library choice: MFMA split-bf16 for kernels >= 15x15, else FP32 VALU */
+  IFM_PREC_AUTO = 0,    /* library choice: matrix cores for 2-D kernels >= 15x15, else FP32 VALU */
   IFM_PREC_FP32 = 1,    /* FP32 FMA on the vector ALUs */
-  IFM_PREC_BF16X3 = 2   /* MFMA, x and k split in bf16 hi+lo, 3 products, FP32 accumulate */
+  IFM_PREC_F16X3 = 2    /* matrix cores: scaled x and k split in FP16 hi+lo, 3 products, FP32
+                           accumulate (2-D kernels up to 49 x 32; error ~2^-22 relative) */
 } ifm_precision;
 
 typedef struct {
@@ -178,6 +179,62 @@ int ifm_mapwindow_minmax(ifm_ctx *ctx, const ifm_array *in, ifm_array *omin, ifm
  * widths allowed), independent of any border. */
 int ifm_extrema_filter(ifm_ctx *ctx, const ifm_array *in, ifm_array *omin, ifm_array *omax,
                        const int64_t *window);
+
+/* ---- batched serving ------------------------------------------------------- */
+/* No counterpart in the reference: the serving form of the filters above, for
+ * many 2-D images of uneven sizes in one launch per op class. */
+typedef enum {
+  IFM_OP_SEPARABLE = 0,  /* imfilter(img, (f1[, f2]), border): factors on axes 0 / 1, <= 64 taps each */
+  IFM_OP_DENSE = 1,      /* imfilter(img, kernel, border, Algorithm.FIR()) on the matrix cores, <= 49 x 32 */
+  IFM_OP_MEDIAN = 2      /* mapwindow(median!, img, (k, k); border), k = 3, 5 or 7 */
+} ifm_op;
+
+/* One request: a 2-D image filtered over its whole extent (out.size == in.size).
+ * With in.data == NULL (ifm_serve only) the loop allocates in and out from its
+ * HBM arena and fills in with seeded uniform [0,1) values (seed = the request's
+ * index), the synthetic input of the serving benchmark. */
+typedef struct {
+  int32_t op;                   /* ifm_op */
+  int32_t nfactors;             /* IFM_OP_SEPARABLE: 1 or 2 */
+  ifm_array in;                 /* device, ndim 2 */
+  ifm_array out;                /* device, ndim 2, same size */
+  ifm_border border;
+  ifm_factor factors[2];        /* IFM_OP_SEPARABLE (host taps) */
+  ifm_kernel kernel;            /* IFM_OP_DENSE (host taps) */
+  int64_t window;               /* IFM_OP_MEDIAN: k */
+  int32_t nfactors2;            /* IFM_OP_SEPARABLE: > 0 adds out2 = imfilter(in, factors2), the */
+  int32_t reserved;             /*   second component of imgradients (src/specialty.jl:39-53) */
+  ifm_array out2;
+  ifm_factor factors2[2];
+} ifm_request;
+
+/* Queues n requests on the ctx stream: one launch per op class (separable,
+ * matrix-core dense per kernel shape, median per window and key width);
+ * circular or smaller-than-window medians run the per-image median kernel.
+ * Results equal those of the single-image entry points' kernels. */
+int ifm_batch_run(ifm_ctx *ctx, const ifm_request *reqs, int n);
+
+typedef struct {
+  uint64_t budget_bytes;        /* HBM admission budget: in + out bytes of the requests in flight */
+  int32_t mode;                 /* 0: continuous batching, 1: per-call dispatch (one synchronous call each) */
+  int32_t max_batch;            /* most requests admitted into one batch (0: 1024) */
+  int32_t max_inflight;         /* most batches queued on the device at once (0: 4) */
+  int32_t reserved;
+} ifm_serve_config;
+
+/* Device-resident request loop.  Request i arrives arrival_s[i] seconds after
+ * the call starts (non-decreasing).  Mode 0 admits arrived requests in arrival
+ * order while their bytes fit the budget, launches them as a batch without
+ * waiting for earlier batches, and retires batches as their events complete.
+ * Mode 1 runs each request alone: allocate, generate, the single-image entry
+ * point, synchronise.  On return admit_s[i] / done_s[i] hold the times (s,
+ * since the call started) at which request i was launched / seen complete. */
+int ifm_serve(ifm_ctx *ctx, const ifm_request *reqs, const double *arrival_s, int n,
+              const ifm_serve_config *cfg, double *admit_s, double *done_s);
+
+/* Fills a device array with seeded uniform [0,1) values: the top 24 bits of a
+ * counter hash times 2^-24 (exact in FP32; BF16 / FP16 round to nearest even). */
+int ifm_fill_uniform(ifm_ctx *ctx, ifm_array *a, uint64_t seed);
 
 /* ---- device memory plumbing ------------------------------------------------ */
 /* Helpers for hosts without their own HIP allocator (the Python package and

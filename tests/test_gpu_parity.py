@@ -209,6 +209,94 @@ def test_dense_mfma_split_bf16(K):
         assert e <= TOL, (K, b, e)
 
 
+@pytest.mark.parametrize("K,dtype", [(15, "f32"), (15, "bf16"), (15, "u8"), (15, "f64"), (31, "f32")])
+def test_dense_mfma_many_tiles_per_workgroup(K, dtype):
+    """A stack of planes gives every persistent workgroup several tiles, so the
+    prefetch of the next tile (and, for K = 31, the rotation of the Toeplitz
+    register sets) is exercised; plane and border changes happen mid-run."""
+    rng = np.random.default_rng(40 + K)
+    shape = (700, 300, 14) if K == 15 else (400, 260, 8)
+    x = rng.random(shape).astype(np.float32)
+    if dtype == "bf16":
+        img = IF.bf16(x)
+        x64 = IF.bf16_to_float32(img).astype(np.float64)
+    elif dtype == "u8":
+        img = (x * 255).astype(np.uint8)
+        x64 = img.astype(np.float64)
+    elif dtype == "f64":
+        img = x.astype(np.float64)
+        x64 = img
+    else:
+        img = x
+        x64 = x.astype(np.float64)
+    k = norm_random_kernel(rng, K)[:, :, None]
+    kern = IF.centered(k)
+    fill = IF.Fill(3) if dtype == "u8" else IF.Fill(0.25)  # Fill converts to the image eltype
+    for b in ["reflect", fill]:
+        out = IF.imfilter(img, kern, b, IF.Algorithm.FIR())
+        want = O.imfilter_dense(x64, k, list(kern.first), obord(b))
+        e = relerr(out, want)
+        assert e <= TOL, (K, dtype, b, e)
+
+
+def _dense_lib(img, k, first, border, precision):
+    """ifm_imfilter_dense with an explicit precision on device arrays."""
+    from imagefiltering import _lib
+    from imagefiltering.device import ifm_array
+    from imagefiltering.imfilter import _border_struct, _kernel_struct
+    L, ctx = _lib.load(), _lib.ctx()
+    d_in = IF.DeviceArray.from_host(img)
+    d_out = d_in.similar()
+    rc = L.ifm_imfilter_dense(ctx, ifm_array(d_in), ifm_array(d_out), None,
+                              _kernel_struct(IF.OffsetArray(k, *first)), _border_struct(border, IF.Float32),
+                              precision)
+    _lib.check(rc)
+    return d_out.to_host()
+
+
+@pytest.mark.parametrize("ks", [(3, 3), (5, 9), (25, 5), (17, 16), (49, 32), (33, 20)])
+def test_dense_mfma_explicit_shapes(ks):
+    from imagefiltering import _lib
+    rng = np.random.default_rng(50 + ks[0] * 7 + ks[1])
+    # column-major, so kernel axis 0 stays the fast image axis (a row-major
+    # image transposes the kernel in the library's canonical layout)
+    img = np.asfortranarray(rng.random((333, 290)).astype(np.float32))
+    k = rng.standard_normal(ks)
+    k /= np.abs(k).sum()
+    first = [-(ks[0] // 2), -(ks[1] // 3)]
+    for b in ["symmetric", "circular", IF.Fill(-1.0)]:
+        out = _dense_lib(img, k, first, b if isinstance(b, IF.Fill) else IF.Pad(b), _lib.PREC_F16X3)
+        want = O.imfilter_dense(img.astype(np.float64), k, first, obord(b))
+        e = relerr(out, want)
+        assert e <= TOL, (ks, b, e)
+
+
+def test_dense_mfma_rejects_unsupported_shape():
+    from imagefiltering import _lib
+    img = np.asfortranarray(np.ones((64, 64), dtype=np.float32))
+    with pytest.raises(NotImplementedError):
+        _dense_lib(img, np.ones((3, 40)), [0, 0], IF.Pad("replicate"), _lib.PREC_F16X3)
+
+
+def test_dense_mfma_nonfinite_tiles():
+    """NaN and Inf reach exactly the outputs whose window holds them (tiles with
+    a non-finite value take the direct FP32 path)."""
+    rng = np.random.default_rng(60)
+    img = rng.random((520, 400)).astype(np.float32)
+    img[37, 211] = np.nan
+    img[300, 5] = np.inf
+    img[450, 390] = -np.inf
+    k = norm_random_kernel(rng, 15)
+    kern = IF.centered(k)
+    out = IF.imfilter(img, kern, "replicate", IF.Algorithm.FIR())
+    want = O.imfilter_dense(img.astype(np.float64), k, list(kern.first), "replicate")
+    np.testing.assert_array_equal(np.isnan(out), np.isnan(want))
+    np.testing.assert_array_equal(np.isinf(out), np.isinf(want))
+    fin = np.isfinite(want)
+    assert fin.sum() > 0.9 * want.size
+    assert relerr(np.where(fin, out, 0), np.where(fin, want, 0)) <= TOL
+
+
 def test_dense_mfma_bf16_storage_and_log():
     rng = np.random.default_rng(18)
     x = rng.random((200, 190)).astype(np.float32)

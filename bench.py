@@ -65,6 +65,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=20000)
     ap.add_argument("--profile-copy", action="store_true",
                     help="also run one plain copy of a known size (rocprof counter calibration)")
     args = ap.parse_args()
@@ -131,8 +132,11 @@ def main():
                                                                                           err))
 
     extras = {}
+    serving = None
     if not args.no_extras:
+        del out
         extras = run_extras(IF, _lib, L, ctx, T, img, d_img, d_out, b_reflect, args)
+        serving = run_serving(args)
     if args.profile_copy:
         # a plain f32 copy of a known size: 256 MiB read + 256 MiB written
         m = 1 << 26
@@ -174,9 +178,38 @@ def main():
                          "sample": "rows %d..%d (%d x 4096 px) of the same image and kernel, FP64 direct FIR "
                                    "(oracle/ifm_oracle.c, single thread like CPU1(Algorithm.FIR()))"
                                    % (r0, r0 + nrows - 1, nrows)},
+        "serving": serving,
         "extras": extras,
     }
     print(json.dumps(rec), flush=True)
+
+
+def run_serving(args):
+    """The serving mix: the same seeded request stream through the batched
+    request loop and through plain per-call dispatch, first closed-loop (all
+    requests arrive at once: saturation throughput), then open-loop with
+    Poisson arrivals at 0.8 x the batched saturation rate (latency)."""
+    from imagefiltering import serving
+    n = args.serve_requests
+    budget = 64 << 30
+    res = {"requests": n, "hbm_budget_bytes": budget,
+           "mix": "sides log-uniform [64,4096]; F32 or BF16; gaussian sigma~U[0.5,8] | sobel gradient pair | "
+                  "median 7x7 | dense 15x15; five borders; inputs generated on the device"}
+    mix = serving.request_mix(n, rate=None)
+    for mode in ("batched", "percall"):
+        _, done = serving.serve(mix, mode, budget_bytes=budget)
+        res["closed_" + mode] = serving.stats(mix, done)
+        log("serving closed-loop %-8s %s" % (mode, res["closed_" + mode]))
+    sat = res["closed_batched"]["images_per_s"]
+    rate = 0.8 * sat
+    mix = serving.request_mix(n, rate=rate)
+    res["open_rate_per_s"] = rate
+    for mode in ("batched", "percall"):
+        _, done = serving.serve(mix, mode, budget_bytes=budget)
+        res["open_" + mode] = serving.stats(mix, done)
+        log("serving open-loop %.0f/s %-8s %s" % (rate, mode, res["open_" + mode]))
+    res["speedup_images_per_s"] = sat / res["closed_percall"]["images_per_s"]
+    return res
 
 
 def run_extras(IF, _lib, L, ctx, T, img, d_img, d_out, b_reflect, args):

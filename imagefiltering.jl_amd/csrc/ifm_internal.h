@@ -214,6 +214,15 @@ bool dense_mfma_args(const Dense4 &in, Dense4 &out, const int64_t *out_first, co
                      const uint16_t *d_table, const float *d_taps, float kinv, int style, float fill, MfmaArgs *a);
 // Kernel variant of a kernel shape (equal for requests that can share a launch).
 int dense_mfma_variant(const DenseKernel &k);
+// Device-built Toeplitz tables (same bits as dense_mfma_table).
+struct ToeplitzJob {
+  const float *taps;  // device FP32 taps
+  uint16_t *table;    // device table
+  int kx, ky, kb, urows;
+  float ks;
+};
+size_t dense_mfma_job(const DenseKernel &k, ToeplitzJob *job, float *kinv);  // returns the table bytes
+hipError_t launch_toeplitz(const ToeplitzJob *jobs, int njobs, hipStream_t st);
 hipError_t launch_dense_mfma(const Dense4 &in, Dense4 &out, const int64_t *out_first, const DenseKernel &k,
                              const uint16_t *d_table, const float *d_taps, float kinv, int style, float fill,
                              hipStream_t st);
@@ -242,6 +251,21 @@ struct MedianArgs {
 };
 hipError_t launch_median(const Dense4 &in, Dense4 &out, const int64_t *out_first, const int *wfirst,
                          const int *wlen, int style, double fill, hipStream_t st);
+// The circular edge band of a centred k x k median (after the network kernel).
+hipError_t median_circular_band(const Dense4 &in, Dense4 &out, const int64_t *out_first, int k, double fill,
+                                hipStream_t st);
+struct BandJob {  // one band strip (out indices) of a circular median
+  const void *in;
+  void *out;
+  int64_t in_plane, out_plane;
+  int n0, n1, nplanes, opitch;
+  int f0, f1;
+  int x0, y0, w, h;
+  int dt;
+};
+int median_band_jobs(const Dense4 &in, const Dense4 &out, const int64_t *out_first, int k, BandJob *jobs);
+hipError_t launch_median_band_batch(int k, const BandJob *jobs, const int *tile_req, const int *tile_first, int ntiles,
+                                    hipStream_t st);
 // Separable min/max over truncated windows; with fill, the fill key joins any
 // window that leaves the image.  key buffers: two arrays of count() uint32/64.
 hipError_t launch_minmax(const Dense4 &in, Dense4 *omin, Dense4 *omax, const int64_t *out_first,

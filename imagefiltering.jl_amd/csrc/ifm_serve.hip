@@ -215,7 +215,7 @@ DenseKernel dense_of(const ifm_kernel &k) {
 
 // Launch list of one batch, with offsets into the descriptor block.
 struct Group {
-  int kind;     // 0 fill, 1 separable, 2 median, 3 dense
+  int kind;     // 0 fill, 1 separable, 2 median, 3 dense, 4 circular median band, 5 Toeplitz tables
   int param;    // median: k * 2 + pack; dense: variant
   size_t reqs, tile_req, tile_first;
   int ntiles;
@@ -224,8 +224,15 @@ struct Group {
 
 // Builds the descriptor block of requests reqs[idx[0..n)] and their launch
 // list.  gen[i] != 0: member i's input is generated first (seed idx[i]).
+// Medians outside the batched launches: `alone` run the single-image kernel,
+// `band` get their circular edge band recomputed after the batch.
+struct Extra {
+  std::vector<int> alone, band;
+};
+
 int build_batch(ifm_ctx *ctx, const ifm_request *reqs, const int *idx, int n, const char *gen, Block &blk,
-                std::vector<Group> &groups, std::vector<int> &fallback) {
+                std::vector<Group> &groups, Extra &ex) {
+  std::vector<int> &fallback = ex.alone, &band = ex.band;
   std::vector<Dense4> vin(n), vout(n);
   for (int i = 0; i < n; ++i) {
     int rc = check_request(ctx, reqs[idx[i]], vin[i], vout[i]);
@@ -319,7 +326,8 @@ int build_batch(ifm_ctx *ctx, const ifm_request *reqs, const int *idx, int n, co
       tiles_group(1, 0, mem, nt, off, sizeof(float) * sep2d_lds_floats(maxk0, maxk1));
     }
   }
-  // Median, per (window, key width); circular, small or F64 images run alone.
+  // Median, per (window, key width); small or F64 images run alone, and the
+  // edge band of circular ones is recomputed after the batch (band list).
   for (int k = 3; k <= 7; k += 2)
     for (int pack = 0; pack < 2; ++pack) {
       std::vector<int> mem, nt;
@@ -327,12 +335,12 @@ int build_batch(ifm_ctx *ctx, const ifm_request *reqs, const int *idx, int n, co
       for (int i = 0; i < n; ++i) {
         const ifm_request &q = reqs[idx[i]];
         if (q.op != IFM_OP_MEDIAN || q.window != k) continue;
-        const bool net = q.border.style != IFM_CIRCULAR && vin[i].n[0] >= k && vin[i].n[1] >= k &&
-                         vin[i].dtype != IFM_F64;
+        const bool net = vin[i].n[0] >= k && vin[i].n[1] >= k && vin[i].dtype != IFM_F64;
         if (!net) {
           if (pack == 0) fallback.push_back(i);
           continue;
         }
+        if (pack == 0 && q.border.style == IFM_CIRCULAR) band.push_back(i);
         if ((vin[i].dtype != IFM_F32) != (pack == 1)) continue;
         MedianArgs a = {};
         a.in = vin[i].data;
@@ -364,16 +372,20 @@ int build_batch(ifm_ctx *ctx, const ifm_request *reqs, const int *idx, int n, co
         return sfail(ctx, IFM_E_UNSUPPORTED, "dense requests need a 2-D kernel of at most 49 x 32 taps");
       byvar[dense_mfma_variant(dks[i])].push_back(i);
     }
+    // Toeplitz tables are built on the device by one launch ahead of the dense ones.
+    std::vector<ToeplitzJob> tjobs;
+    std::vector<std::pair<size_t, size_t>> tjoboffs;  // (taps, table) offsets
     for (auto &kv : byvar) {
       std::vector<MfmaArgs> ma;
       std::vector<int> nt;
       std::vector<std::pair<size_t, size_t>> offs;  // (table, taps)
       for (int i : kv.second) {
-        std::vector<uint16_t> tab;
+        ToeplitzJob tj = {};
         float kinv = 1.f;
-        dense_mfma_table(dks[i], tab, &kinv);
-        offs.push_back({blk.put(tab.data(), tab.size() * sizeof(uint16_t)),
-                        blk.put(dks[i].taps.data(), dks[i].taps.size() * sizeof(float), 16)});
+        const size_t tbytes = dense_mfma_job(dks[i], &tj, &kinv);
+        offs.push_back({blk.reserve(tbytes), blk.put(dks[i].taps.data(), dks[i].taps.size() * sizeof(float), 16)});
+        tjobs.push_back(tj);
+        tjoboffs.push_back({offs.back().second, offs.back().first});
         MfmaArgs a;
         const int64_t of[MAXD] = {0, 0, 0, 0};
         const ifm_request &q = reqs[idx[i]];
@@ -390,13 +402,46 @@ int build_batch(ifm_ctx *ctx, const ifm_request *reqs, const int *idx, int n, co
       }
       tiles_group(3, kv.first, kv.second, nt, off, 0);
     }
+    if (!tjobs.empty()) {
+      const size_t off = blk.put(tjobs.data(), tjobs.size() * sizeof(ToeplitzJob));
+      for (size_t m = 0; m < tjobs.size(); ++m) {
+        blk.ptr(off + m * sizeof(ToeplitzJob) + offsetof(ToeplitzJob, taps), tjoboffs[m].first);
+        blk.ptr(off + m * sizeof(ToeplitzJob) + offsetof(ToeplitzJob, table), tjoboffs[m].second);
+      }
+      Group g = {5, 0, off, 0, 0, (int)tjobs.size(), 0};
+      // before the first dense launch of this batch
+      auto first_dense = std::find_if(groups.begin(), groups.end(), [](const Group &x) { return x.kind == 3; });
+      groups.insert(first_dense, g);
+    }
   }
+  // Circular median edge bands, per window, after the network launches.
+  for (int k = 3; k <= 7; k += 2) {
+    std::vector<BandJob> jobs;
+    std::vector<int> mem, nt;
+    for (int i : band) {
+      if (reqs[idx[i]].window != k) continue;
+      BandJob bj[4];
+      const int64_t of[MAXD] = {0, 0, 0, 0};
+      const int nj = median_band_jobs(vin[i], vout[i], of, k, bj);
+      for (int s = 0; s < nj; ++s) {
+        mem.push_back((int)jobs.size());
+        nt.push_back((bj[s].w * bj[s].h + 255) / 256);
+        jobs.push_back(bj[s]);
+      }
+    }
+    if (!jobs.empty()) {
+      std::vector<int> ids(jobs.size());
+      for (size_t m = 0; m < ids.size(); ++m) ids[m] = (int)m;
+      tiles_group(4, k, ids, nt, blk.put(jobs.data(), jobs.size() * sizeof(BandJob)), 0);
+    }
+  }
+  band.clear();  // handled by the launches above
   return IFM_OK;
 }
 
 // Copies the block into the next staging slot and queues every launch.
 int launch_batch(ifm_ctx *ctx, const ifm_request *reqs, const int *idx, Block &blk, const std::vector<Group> &groups,
-                 const std::vector<int> &fallback, hipEvent_t *done) {
+                 const Extra &ex, hipEvent_t *done) {
   if (ctx->bslots.size() < NSLOTS) ctx->bslots.resize(NSLOTS);
   ifm_ctx::BatchSlot &s = ctx->bslots[ctx->bnext];
   ctx->bnext = (ctx->bnext + 1) % NSLOTS;
@@ -432,21 +477,29 @@ int launch_batch(ifm_ctx *ctx, const ifm_request *reqs, const int *idx, Block &b
       SHIP(launch_median_batch(g.param / 2, g.param % 2 == 1, (const MedianArgs *)(d + g.reqs), treq, tfirst,
                                g.ntiles, ctx->stream));
       break;
-    default:
+    case 3:
       SHIP(launch_dense_mfma_batch(g.param, (const MfmaArgs *)(d + g.reqs), treq, tfirst, g.ntiles, ctx->stream));
+      break;
+    case 5: SHIP(launch_toeplitz((const ToeplitzJob *)(d + g.reqs), g.ntiles, ctx->stream)); break;
+    default:
+      SHIP(launch_median_band_batch(g.param, (const BandJob *)(d + g.reqs), treq, tfirst, g.ntiles, ctx->stream));
       break;
     }
   }
-  for (int i : fallback) {
-    const ifm_request &q = reqs[idx[i]];
-    Dense4 in, out;
-    int rc = check_request(ctx, q, in, out);
-    if (rc) return rc;
-    const int k = (int)q.window;
-    const int64_t of[MAXD] = {0, 0, 0, 0};
-    const int wf[MAXD] = {-(k / 2), -(k / 2), 0, 0}, wl[MAXD] = {k, k, 1, 1};
-    SHIP(launch_median(in, out, of, wf, wl, q.border.style, q.border.fill, ctx->stream));
-  }
+  for (int pass = 0; pass < 2; ++pass)
+    for (int i : pass ? ex.band : ex.alone) {
+      const ifm_request &q = reqs[idx[i]];
+      Dense4 in, out;
+      int rc = check_request(ctx, q, in, out);
+      if (rc) return rc;
+      const int k = (int)q.window;
+      const int64_t of[MAXD] = {0, 0, 0, 0};
+      const int wf[MAXD] = {-(k / 2), -(k / 2), 0, 0}, wl[MAXD] = {k, k, 1, 1};
+      if (pass)
+        SHIP(median_circular_band(in, out, of, k, q.border.fill, ctx->stream));
+      else
+        SHIP(launch_median(in, out, of, wf, wl, q.border.style, q.border.fill, ctx->stream));
+    }
   SHIP(hipEventRecord(s.ev, ctx->stream));
   s.pending = true;
   if (done) *done = s.ev;
@@ -509,10 +562,10 @@ int ifm_batch_run(ifm_ctx *ctx, const ifm_request *reqs, int n) {
   for (int i = 0; i < n; ++i) idx[i] = i;
   Block blk;
   std::vector<Group> groups;
-  std::vector<int> fallback;
-  int rc = build_batch(ctx, reqs, idx.data(), n, nullptr, blk, groups, fallback);
+  Extra ex;
+  int rc = build_batch(ctx, reqs, idx.data(), n, nullptr, blk, groups, ex);
   if (rc) return rc;
-  return launch_batch(ctx, reqs, idx.data(), blk, groups, fallback, nullptr);
+  return launch_batch(ctx, reqs, idx.data(), blk, groups, ex, nullptr);
 }
 
 int ifm_fill_uniform(ifm_ctx *ctx, ifm_array *a, uint64_t seed) {
@@ -652,16 +705,16 @@ int ifm_serve(ifm_ctx *ctx, const ifm_request *reqs, const double *arrival_s, in
         if (!members.empty()) {
           Block blk;
           std::vector<Group> groups;
-          std::vector<int> fallback;
+          Extra ex;
           std::vector<char> genf(members.size());
           for (size_t m = 0; m < members.size(); ++m) genf[m] = !reqs[members[m]].in.data;
           rc = build_batch(ctx, rq.data(), members.data(), (int)members.size(), gen ? genf.data() : nullptr, blk,
-                           groups, fallback);
+                           groups, ex);
           if (!rc) {
             const double t = now();
             for (int i : members) admit_s[i] = t;
             Flight f;
-            rc = launch_batch(ctx, rq.data(), members.data(), blk, groups, fallback, &f.ev);
+            rc = launch_batch(ctx, rq.data(), members.data(), blk, groups, ex, &f.ev);
             f.members = members;
             if (!rc) flights.push_back(f);
           }

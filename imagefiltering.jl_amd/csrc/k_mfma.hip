@@ -635,30 +635,73 @@ bool dense_mfma_supported(const DenseKernel &k, const Dense4 &in) {
 
 // Host-side Toeplitz fragment table (FP16 hi/lo of the scaled taps), padded
 // to the chunk count the kernel runs.
-void dense_mfma_table(const DenseKernel &k, std::vector<uint16_t> &tab, float *kinv) {
-  const int kx = k.n[0], ky = k.n[1];
-  const int kb = mfma_kb(kx), nch = mfma_nch(ky);
+// Power-of-two scale putting the kernel's largest |tap| in [0.5, 1).
+static void mfma_scale(const DenseKernel &k, float *ks, float *kinv) {
   float amax = 0.f;
   for (float t : k.taps) amax = fmaxf(amax, fabsf(t));
   int e = 0;
   if (amax > 0.f) frexpf(amax, &e);
-  const float ks = ldexpf(1.0f, -e);
+  *ks = ldexpf(1.0f, -e);
   *kinv = ldexpf(1.0f, e);
-  const int urows = nch * MF_UCH;
-  tab.assign((size_t)kb * urows * 2 * 64 * 8, 0);
-  for (int b = 0; b < kb; ++b)
-    for (int u = 0; u < urows; ++u)
-      for (int lane = 0; lane < 64; ++lane)
-        for (int i = 0; i < 8; ++i) {
-          const int s = 32 * b + 8 * (lane >> 4) + i;
-          const int t = s - (lane & 15);
-          const float kv = (t >= 0 && t < kx && u < ky) ? k.taps[t + u * kx] * ks : 0.f;
-          const _Float16 h = (_Float16)kv;
-          const _Float16 l = (_Float16)(kv - (float)h);
-          const size_t base = ((((size_t)b * urows + u) * 2) * 64 + lane) * 8 + i;
-          tab[base] = __builtin_bit_cast(uint16_t, h);
-          tab[base + 64 * 8] = __builtin_bit_cast(uint16_t, l);
-        }
+}
+
+// Table entry e (of kb * urows * 64 * 8): FP16 hi and lo of the scaled tap
+// T_u[s][a] = k[s - a, u] for K block b, kernel row u, lane (a = lane % 16,
+// s = 32 b + 8 (lane / 16) + i) and element i; returns its position.
+__host__ __device__ inline size_t toeplitz_entry(const float *taps, int kx, int ky, int urows, float ks, int e,
+                                                 uint16_t *hi, uint16_t *lo) {
+  const int i = e & 7, lane = (e >> 3) & 63, bu = e >> 9;
+  const int u = bu % urows, b = bu / urows;
+  const int s = 32 * b + 8 * (lane >> 4) + i;
+  const int t = s - (lane & 15);
+  const float kv = (t >= 0 && t < kx && u < ky) ? taps[t + u * kx] * ks : 0.f;
+  const _Float16 h = (_Float16)kv;
+  const _Float16 l = (_Float16)(kv - (float)h);
+  *hi = __builtin_bit_cast(uint16_t, h);
+  *lo = __builtin_bit_cast(uint16_t, l);
+  return ((((size_t)b * urows + u) * 2) * 64 + lane) * 8 + i;
+}
+
+void dense_mfma_table(const DenseKernel &k, std::vector<uint16_t> &tab, float *kinv) {
+  const int kx = k.n[0], ky = k.n[1];
+  const int kb = mfma_kb(kx), urows = mfma_nch(ky) * MF_UCH;
+  float ks;
+  mfma_scale(k, &ks, kinv);
+  const int entries = kb * urows * 64 * 8;
+  tab.assign((size_t)entries * 2, 0);
+  for (int e = 0; e < entries; ++e) {
+    uint16_t h, l;
+    const size_t pos = toeplitz_entry(k.taps.data(), kx, ky, urows, ks, e, &h, &l);
+    tab[pos] = h;
+    tab[pos + 64 * 8] = l;
+  }
+}
+
+// The same tables built on the device, one workgroup per job (batched serving).
+__global__ void __launch_bounds__(256) k_toeplitz(const ToeplitzJob *__restrict__ jobs) {
+  const ToeplitzJob &j = jobs[blockIdx.x];
+  const int entries = j.kb * j.urows * 64 * 8;
+  for (int e = threadIdx.x; e < entries; e += 256) {
+    uint16_t h, l;
+    const size_t pos = toeplitz_entry(j.taps, j.kx, j.ky, j.urows, j.ks, e, &h, &l);
+    j.table[pos] = h;
+    j.table[pos + 64 * 8] = l;
+  }
+}
+
+size_t dense_mfma_job(const DenseKernel &k, ToeplitzJob *job, float *kinv) {
+  job->kx = k.n[0];
+  job->ky = k.n[1];
+  job->kb = mfma_kb(k.n[0]);
+  job->urows = mfma_nch(k.n[1]) * MF_UCH;
+  mfma_scale(k, &job->ks, kinv);
+  return (size_t)job->kb * job->urows * 64 * 8 * 2 * sizeof(uint16_t);
+}
+
+hipError_t launch_toeplitz(const ToeplitzJob *jobs, int njobs, hipStream_t st) {
+  if (njobs == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_toeplitz, dim3(njobs), dim3(256), 0, st, jobs);
+  return hipGetLastError();
 }
 
 int dense_mfma_variant(const DenseKernel &k) { return mfma_kb(k.n[0]) * 16 + mfma_nch(k.n[1]); }

@@ -154,6 +154,8 @@ struct MedianAnyArgs {
   double fill;
   int64_t total;
   int L;
+  int64_t obase;             // output element of region index 0
+  int64_t ostride[MAXD];     // output strides (elements) along the region's axes
 };
 
 template <typename KT>
@@ -201,11 +203,13 @@ __global__ void __launch_bounds__(256) k_median_any(MedianAnyArgs a) {
   const int rank = (a.L + 1) / 2 - 1;
   for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < a.total;
        p += (int64_t)gridDim.x * blockDim.x) {
-    int64_t r = p;
+    int64_t r = p, oidx = a.obase;
     int q[MAXD], ilo[MAXD], ilen[MAXD];
 #pragma unroll
     for (int d = 0; d < MAXD; ++d) {
-      q[d] = (int)(r % a.o[d]) + a.of[d];
+      const int ql = (int)(r % a.o[d]);
+      oidx += ql * a.ostride[d];
+      q[d] = ql + a.of[d];
       r /= a.o[d];
       int lo = q[d] + a.wf[d], hi = q[d] + a.wf[d] + a.wl[d] - 1;
       ilo[d] = lo < 0 ? 0 : lo;
@@ -226,8 +230,8 @@ __global__ void __launch_bounds__(256) k_median_any(MedianAnyArgs a) {
         break;
       }
     }
-    if constexpr (sizeof(KT) == 8) store_key64(a.out, p, result);
-    else store_key32(a.out, p, a.dt, result);
+    if constexpr (sizeof(KT) == 8) store_key64(a.out, oidx, result);
+    else store_key32(a.out, oidx, a.dt, result);
   }
 }
 
@@ -303,18 +307,179 @@ hipError_t launch_median_batch(int k, bool pack, const MedianArgs *reqs, const i
 // eltype, as an order key).
 uint32_t median_fill_key(int dt, double fill) { return fill_key32(dt, fill); }
 
+// Generic median of the out region [rf, rf + rn) (out indices) of out.
+static hipError_t median_any_region(const Dense4 &in, Dense4 &out, const int64_t *out_first, const int64_t *rf,
+                                    const int64_t *rn, const int *wfirst, const int *wlen, int style, double fill,
+                                    hipStream_t st) {
+  MedianAnyArgs a;
+  a.in = in.data;
+  a.out = out.data;
+  a.L = 1;
+  a.total = 1;
+  a.obase = 0;
+  int64_t stride = 1;
+  for (int d = 0; d < MAXD; ++d) {
+    a.n[d] = (int)in.n[d];
+    a.o[d] = (int)rn[d];
+    a.of[d] = (int)(out_first[d] + rf[d]);
+    a.wf[d] = wfirst[d];
+    a.wl[d] = wlen[d];
+    a.L *= wlen[d];
+    a.total *= rn[d];
+    a.ostride[d] = stride;
+    a.obase += rf[d] * stride;
+    stride *= out.n[d];
+  }
+  if (a.total == 0) return hipSuccess;
+  a.style = style;
+  a.dt = in.dtype;
+  a.fill = fill;
+  int64_t blocks = (a.total + 255) / 256;
+  if (blocks > 1 << 20) blocks = 1 << 20;
+  if (in.dtype == IFM_F64)
+    hipLaunchKernelGGL(k_median_any<uint64_t>, dim3((unsigned)blocks), dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL(k_median_any<uint32_t>, dim3((unsigned)blocks), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+// Circular borders: mapwindow wraps an edge window inside the window's
+// intersection with the image (src/mapwindow.jl:310-316, padindex of
+// src/border.jl:592-596 on win ∩ img), so outputs whose k x k window leaves
+// the image are recomputed after the network kernel has written every output
+// (the interior is style-independent).  One thread per band output gathers
+// its k*k keys into registers through the window-relative map and selects the
+// ((L+1)÷2)-th smallest by counting.
+template <int K>
+__device__ __forceinline__ void band_one(const BandJob &j, int local, int plane) {
+  constexpr int H = K / 2, L = K * K;
+  if (local >= j.w * j.h) return;
+  const int ox = j.x0 + local % j.w, oy = j.y0 + local / j.w;
+  const int x = j.f0 + ox, y = j.f1 + oy;
+  const int xlo = max(0, x - H), xm = min(j.n0 - 1, x + H) - xlo + 1;
+  const int ylo = max(0, y - H), ym = min(j.n1 - 1, y + H) - ylo + 1;
+  const int64_t ibase = (int64_t)plane * j.in_plane;
+  uint32_t k[L];
+#pragma unroll
+  for (int v = 0; v < K; ++v) {
+    int yy = y - H + v;
+    if (yy < ylo || yy >= ylo + ym) yy = ylo + imod(yy - ylo, ym);
+#pragma unroll
+    for (int u = 0; u < K; ++u) {
+      int xx = x - H + u;
+      if (xx < xlo || xx >= xlo + xm) xx = xlo + imod(xx - xlo, xm);
+      k[v * K + u] = key32(j.in, ibase + (int64_t)yy * j.n0 + xx, j.dt);
+    }
+  }
+  constexpr int rank = (L + 1) / 2 - 1;
+  uint32_t res = k[0];
+#pragma unroll
+  for (int i = 0; i < L; ++i) {
+    int less = 0, eq = 0;
+#pragma unroll
+    for (int t = 0; t < L; ++t) {
+      less += k[t] < k[i];
+      eq += k[t] == k[i];
+    }
+    res = (less <= rank && rank < less + eq) ? k[i] : res;
+  }
+  store_key32(j.out, (int64_t)plane * j.out_plane + (int64_t)oy * j.opitch + ox, j.dt, res);
+}
+
+struct BandJobs4 {
+  BandJob j[4];
+  int first[5];  // first block of each job
+};
+
+template <int K>
+__global__ void __launch_bounds__(256) k_median_band4(BandJobs4 jobs) {
+  const int b = blockIdx.x;
+  int s = 0;
+  while (s < 3 && b >= jobs.first[s + 1]) ++s;
+  for (int plane = blockIdx.y; plane < jobs.j[s].nplanes; plane += gridDim.y)
+    band_one<K>(jobs.j[s], (b - jobs.first[s]) * 256 + threadIdx.x, plane);
+}
+
+template <int K>
+__global__ void __launch_bounds__(256) k_median_band_batch(const BandJob *__restrict__ jobs,
+                                                           const int *__restrict__ tile_req,
+                                                           const int *__restrict__ tile_first) {
+  const int g = blockIdx.x, r = tile_req[g];
+  band_one<K>(jobs[r], (g - tile_first[r]) * 256 + threadIdx.x, 0);
+}
+
+// The (up to) four band strips of an out region, in out indices.
+int median_band_jobs(const Dense4 &in, const Dense4 &out, const int64_t *out_first, int k, BandJob *jobs) {
+  const int64_t h = k / 2, o0 = out.n[0], o1 = out.n[1];
+  auto clampi = [](int64_t v, int64_t lo, int64_t hi) { return v < lo ? lo : (v > hi ? hi : v); };
+  const int64_t top = clampi(h - out_first[1], 0, o1), bot = clampi(in.n[1] - h - out_first[1], top, o1);
+  const int64_t left = clampi(h - out_first[0], 0, o0), right = clampi(in.n[0] - h - out_first[0], left, o0);
+  const int64_t rect[4][4] = {{0, 0, o0, top}, {0, bot, o0, o1 - bot}, {0, top, left, bot - top},
+                              {right, top, o0 - right, bot - top}};
+  int n = 0;
+  for (int s = 0; s < 4; ++s) {
+    if (rect[s][2] <= 0 || rect[s][3] <= 0) continue;
+    BandJob &j = jobs[n++];
+    j.in = in.data;
+    j.out = out.data;
+    j.n0 = (int)in.n[0];
+    j.n1 = (int)in.n[1];
+    j.in_plane = in.n[0] * in.n[1];
+    j.out_plane = out.n[0] * out.n[1];
+    j.nplanes = (int)(out.n[2] * out.n[3]);
+    j.opitch = (int)out.n[0];
+    j.f0 = (int)out_first[0];
+    j.f1 = (int)out_first[1];
+    j.x0 = (int)rect[s][0];
+    j.y0 = (int)rect[s][1];
+    j.w = (int)rect[s][2];
+    j.h = (int)rect[s][3];
+    j.dt = in.dtype;
+  }
+  return n;
+}
+
+hipError_t median_circular_band(const Dense4 &in, Dense4 &out, const int64_t *out_first, int k, double fill,
+                                hipStream_t st) {
+  (void)fill;  // circular windows never read the fill value
+  BandJobs4 jobs = {};
+  const int n = median_band_jobs(in, out, out_first, k, jobs.j);
+  if (n == 0) return hipSuccess;
+  int tot = 0;
+  for (int s = 0; s < 4; ++s) {
+    jobs.first[s] = tot;
+    if (s < n) tot += (jobs.j[s].w * jobs.j[s].h + 255) / 256;
+  }
+  jobs.first[4] = tot;
+  for (int s = n; s < 4; ++s) jobs.first[s] = tot;  // empty jobs are never selected
+  const int np = jobs.j[0].nplanes < 65535 ? jobs.j[0].nplanes : 65535;
+  dim3 grid(tot, np);
+  if (k == 3) hipLaunchKernelGGL(k_median_band4<3>, grid, dim3(256), 0, st, jobs);
+  else if (k == 5) hipLaunchKernelGGL(k_median_band4<5>, grid, dim3(256), 0, st, jobs);
+  else hipLaunchKernelGGL(k_median_band4<7>, grid, dim3(256), 0, st, jobs);
+  return hipGetLastError();
+}
+
+hipError_t launch_median_band_batch(int k, const BandJob *jobs, const int *tile_req, const int *tile_first, int ntiles,
+                                    hipStream_t st) {
+  if (ntiles == 0) return hipSuccess;
+  if (k == 3) hipLaunchKernelGGL(k_median_band_batch<3>, dim3(ntiles), dim3(256), 0, st, jobs, tile_req, tile_first);
+  else if (k == 5) hipLaunchKernelGGL(k_median_band_batch<5>, dim3(ntiles), dim3(256), 0, st, jobs, tile_req, tile_first);
+  else hipLaunchKernelGGL(k_median_band_batch<7>, dim3(ntiles), dim3(256), 0, st, jobs, tile_req, tile_first);
+  return hipGetLastError();
+}
+
 hipError_t launch_median(const Dense4 &in, Dense4 &out, const int64_t *out_first, const int *wfirst,
                          const int *wlen, int style, double fill, hipStream_t st) {
   if (out.count() == 0) return hipSuccess;
-  // The tiled network path needs a centred odd square 2-D window and an
-  // image-relative border map (which equals mapwindow's window-relative one
-  // for centred windows except for circular).
+  // The tiled network path needs a centred odd square 2-D window; its
+  // image-relative border map equals mapwindow's window-relative one for such
+  // windows except for circular, whose edge band is recomputed afterwards.
   int k = wlen[0];
   bool square = (k == 3 || k == 5 || k == 7) && wlen[1] == k && wfirst[0] == -(k / 2) && wfirst[1] == -(k / 2) &&
                 wlen[2] == 1 && wlen[3] == 1 && wfirst[2] == 0 && wfirst[3] == 0;
-  bool styleok = style != IFM_CIRCULAR;
   bool small = in.n[0] >= k && in.n[1] >= k;  // mirrored maps stay within the window
-  if (square && styleok && small && in.dtype != IFM_F64) {
+  if (square && small && in.dtype != IFM_F64) {
     MedianArgs a;
     a.in = in.data;
     a.out = out.data;
@@ -342,31 +507,12 @@ hipError_t launch_median(const Dense4 &in, Dense4 &out, const int64_t *out_first
     } else {
       if (pack) launch_net_on<7, 2, 2, 2>(a, st); else launch_net_on<7, 2, 2, 1>(a, st);
     }
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && style == IFM_CIRCULAR) e = median_circular_band(in, out, out_first, k, fill, st);
+    return e;
   }
-  MedianAnyArgs a;
-  a.in = in.data;
-  a.out = out.data;
-  a.L = 1;
-  for (int d = 0; d < MAXD; ++d) {
-    a.n[d] = (int)in.n[d];
-    a.o[d] = (int)out.n[d];
-    a.of[d] = (int)out_first[d];
-    a.wf[d] = wfirst[d];
-    a.wl[d] = wlen[d];
-    a.L *= wlen[d];
-  }
-  a.style = style;
-  a.dt = in.dtype;
-  a.fill = fill;
-  a.total = out.count();
-  int64_t blocks = (a.total + 255) / 256;
-  if (blocks > 1 << 20) blocks = 1 << 20;
-  if (in.dtype == IFM_F64)
-    hipLaunchKernelGGL(k_median_any<uint64_t>, dim3((unsigned)blocks), dim3(256), 0, st, a);
-  else
-    hipLaunchKernelGGL(k_median_any<uint32_t>, dim3((unsigned)blocks), dim3(256), 0, st, a);
-  return hipGetLastError();
+  const int64_t rf[MAXD] = {0, 0, 0, 0};
+  return median_any_region(in, out, out_first, rf, out.n, wfirst, wlen, style, fill, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -493,6 +639,242 @@ __global__ void __launch_bounds__(256) k_minmax_pass(MinmaxPassArgs a, KT fillke
   }
 }
 
+// ---------------------------------------------------------------------------
+// 2-D min / max in one pass: a 64 x 32 output tile loads its (32 + w1 - 1) x
+// (64 + w0 - 1) input keys into LDS with clamped (replicate) coordinates --
+// the min / max over a clamped window equals the one over the window's
+// intersection with the image whenever the window contains its own pixel --
+// then reduces along axis 0 into two LDS tiles and along axis 1 into the
+// output.  With a compile-time window the reductions are log-depth: level s
+// turns m[x] = op(v[x .. x+s)) into op(v[x .. x+2s)), and the window result
+// is op(m[x], m[x + w - p]) for the largest power of two p <= w.
+// ---------------------------------------------------------------------------
+constexpr int MM_TX = 64, MM_TY = 32, MM_MAXW = 33;
+
+struct Minmax2dArgs {
+  const void *in;
+  void *omin, *omax;
+  int n0, n1;            // image
+  int o0, o1;            // output extents
+  int f0, f1;            // image coordinates of output (0, 0)
+  int wf0, wf1, wl0, wl1;
+  int isfill;
+  uint32_t fillkey;
+  int64_t in_plane, out_plane;
+  int nplanes;
+};
+
+template <int W, int NO>
+__device__ __forceinline__ void win_minmax(uint32_t (&vn)[NO + W - 1], uint32_t (&vx)[NO + W - 1], uint32_t (&on)[NO],
+                                           uint32_t (&ox)[NO]) {
+  constexpr int L = NO + W - 1;
+  constexpr int P = W >= 32 ? 32 : W >= 16 ? 16 : W >= 8 ? 8 : W >= 4 ? 4 : W >= 2 ? 2 : 1;
+#pragma unroll
+  for (int s = 1; 2 * s <= P; s *= 2) {
+#pragma unroll
+    for (int x = 0; x + s < L; ++x) {  // m[x+s] is still the previous level's value
+      vn[x] = min(vn[x], vn[x + s]);
+      vx[x] = max(vx[x], vx[x + s]);
+    }
+  }
+#pragma unroll
+  for (int o = 0; o < NO; ++o) {
+    on[o] = min(vn[o], vn[o + W - P]);
+    ox[o] = max(vx[o], vx[o + W - P]);
+  }
+}
+
+template <int DT>
+__device__ __forceinline__ uint32_t mm_minkey(uint32_t k) {
+  if constexpr (DT == IFM_F32) return k == 0xffffffffu ? 0u : k;
+  else if constexpr (DT == IFM_BF16 || DT == IFM_F16) return k == 0xffffu ? 0u : k;
+  else return k;
+}
+
+// DT: IFM_U8 (also N0f8), IFM_BF16, IFM_F16 or IFM_F32; W0 = W1 = 0: runtime window.
+template <int DT, int W0, int W1>
+__global__ void __launch_bounds__(256) k_minmax2d(Minmax2dArgs a) {
+  extern __shared__ uint32_t mm_lds[];
+  const int w0 = W0 ? W0 : a.wl0, w1 = W1 ? W1 : a.wl1;
+  const int iw = MM_TX + w0 - 1, ih = MM_TY + w1 - 1, iwp = iw | 1;
+  uint32_t *s_in = mm_lds;                 // ih x iwp raw keys
+  uint32_t *s_n = mm_lds + ih * iwp;       // ih x 64 min keys after axis 0
+  uint32_t *s_x = s_n + ih * MM_TX;        // ih x 64 max keys after axis 0
+  const int tid = threadIdx.x;
+  const int ox0 = blockIdx.x * MM_TX, oy0 = blockIdx.y * MM_TY;
+  const int gx0 = a.f0 + ox0 + a.wf0, gy0 = a.f1 + oy0 + a.wf1;
+  for (int plane = blockIdx.z; plane < a.nplanes; plane += gridDim.z) {
+    const int64_t ibase = (int64_t)plane * a.in_plane, obase = (int64_t)plane * a.out_plane;
+    // Stage: the thread's column, every other row, 8 loads in flight.
+    {
+      const int c = tid & 127, rh = tid >> 7;
+      const bool colok = c < iw;
+      const int sx = min(max(gx0 + (colok ? c : iw - 1), 0), a.n0 - 1);
+      const int nrow = (ih - rh + 1) >> 1;
+      for (int j0 = 0; j0 < nrow; j0 += 8) {
+        uint32_t v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int jj = j0 + j < nrow ? j0 + j : nrow - 1;
+          const int sy = min(max(gy0 + rh + 2 * jj, 0), a.n1 - 1);
+          v[j] = key32(a.in, ibase + (int64_t)sy * a.n0 + sx, DT);
+        }
+        if (colok) {
+#pragma unroll
+          for (int j = 0; j < 8; ++j)
+            if (j0 + j < nrow) s_in[(rh + 2 * (j0 + j)) * iwp + c] = v[j];
+        }
+      }
+    }
+    __syncthreads();
+    // Axis 0: item = (row, group of 8 outputs).
+    for (int item = tid; item < ih * 8; item += 256) {
+      const int r = item >> 3, c0 = (item & 7) * 8;
+      const uint32_t *src = s_in + r * iwp + c0;
+      uint32_t on[8], ox[8];
+      if constexpr (W0 > 0) {
+        uint32_t vn[8 + W0 - 1], vx[8 + W0 - 1];
+#pragma unroll
+        for (int t = 0; t < 8 + W0 - 1; ++t) {
+          vx[t] = src[t];
+          vn[t] = mm_minkey<DT>(vx[t]);
+        }
+        win_minmax<W0, 8>(vn, vx, on, ox);
+      } else {
+#pragma unroll
+        for (int o = 0; o < 8; ++o) {
+          on[o] = 0xffffffffu;
+          ox[o] = 0u;
+        }
+        for (int t = 0; t < w0 + 7; ++t) {
+          const uint32_t k = src[t], kn = mm_minkey<DT>(k);
+#pragma unroll
+          for (int o = 0; o < 8; ++o)
+            if (t - o >= 0 && t - o < w0) {
+              on[o] = min(on[o], kn);
+              ox[o] = max(ox[o], k);
+            }
+        }
+      }
+#pragma unroll
+      for (int o = 0; o < 8; ++o) {
+        s_n[r * MM_TX + c0 + o] = on[o];
+        s_x[r * MM_TX + c0 + o] = ox[o];
+      }
+    }
+    __syncthreads();
+    // Axis 1: thread = (column, group of 8 rows).
+    {
+      const int col = tid & 63, r0 = (tid >> 6) * 8;
+      uint32_t on[8], ox[8];
+      if constexpr (W1 > 0) {
+        uint32_t vn[8 + W1 - 1], vx[8 + W1 - 1];
+#pragma unroll
+        for (int t = 0; t < 8 + W1 - 1; ++t) {
+          vn[t] = s_n[(r0 + t) * MM_TX + col];
+          vx[t] = s_x[(r0 + t) * MM_TX + col];
+        }
+        win_minmax<W1, 8>(vn, vx, on, ox);
+      } else {
+#pragma unroll
+        for (int o = 0; o < 8; ++o) {
+          on[o] = 0xffffffffu;
+          ox[o] = 0u;
+        }
+        for (int t = 0; t < w1 + 7; ++t) {
+          const uint32_t kn = s_n[(r0 + t) * MM_TX + col], kx = s_x[(r0 + t) * MM_TX + col];
+#pragma unroll
+          for (int o = 0; o < 8; ++o)
+            if (t - o >= 0 && t - o < w1) {
+              on[o] = min(on[o], kn);
+              ox[o] = max(ox[o], kx);
+            }
+        }
+      }
+      const int ox_ = ox0 + col;
+      if (ox_ < a.o0) {
+        const int gx = a.f0 + ox_ + a.wf0;
+        const bool xout = gx < 0 || gx + w0 - 1 > a.n0 - 1;
+#pragma unroll
+        for (int o = 0; o < 8; ++o) {
+          const int oy = oy0 + r0 + o;
+          if (oy >= a.o1) break;
+          uint32_t mn = on[o], mx = ox[o];
+          if (a.isfill) {
+            const int gy = a.f1 + oy + a.wf1;
+            if (xout || gy < 0 || gy + w1 - 1 > a.n1 - 1) {
+              mn = min(mn, mm_minkey<DT>(a.fillkey));
+              mx = max(mx, a.fillkey);
+            }
+          }
+          const int64_t q = obase + (int64_t)oy * a.o0 + ox_;
+          if (a.omin) store_key32(a.omin, q, DT, minkey_back<uint32_t>(mn, DT));
+          if (a.omax) store_key32(a.omax, q, DT, mx);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <int DT, int W>
+static void mm_launch(const Minmax2dArgs &a, dim3 grid, size_t lds, hipStream_t st) {
+  hipLaunchKernelGGL((k_minmax2d<DT, W, W>), grid, dim3(256), lds, st, a);
+}
+
+template <int DT>
+static void mm_dispatch(const Minmax2dArgs &a, dim3 grid, size_t lds, hipStream_t st) {
+  const int w = a.wl0 == a.wl1 ? a.wl0 : 0;
+  switch (w) {
+  case 3: mm_launch<DT, 3>(a, grid, lds, st); break;
+  case 5: mm_launch<DT, 5>(a, grid, lds, st); break;
+  case 7: mm_launch<DT, 7>(a, grid, lds, st); break;
+  case 9: mm_launch<DT, 9>(a, grid, lds, st); break;
+  default: mm_launch<DT, 0>(a, grid, lds, st); break;
+  }
+}
+
+// One-pass 2-D path; false when the shape needs the general passes.
+static bool minmax2d(const Dense4 &in, Dense4 *omin, Dense4 *omax, const int64_t *out_first, const int *wfirst,
+                     const int *wlen, int isfill, double fill, hipStream_t st, hipError_t *err) {
+  Dense4 &ref = omin ? *omin : *omax;
+  if (in.dtype == IFM_F64) return false;
+  for (int d = 2; d < MAXD; ++d)
+    if (wlen[d] != 1 || wfirst[d] != 0 || out_first[d] != 0 || ref.n[d] != in.n[d]) return false;
+  for (int d = 0; d < 2; ++d)
+    if (wlen[d] > MM_MAXW || wfirst[d] > 0 || wfirst[d] + wlen[d] - 1 < 0) return false;  // window holds its pixel
+  Minmax2dArgs a;
+  a.in = in.data;
+  a.omin = omin ? omin->data : nullptr;
+  a.omax = omax ? omax->data : nullptr;
+  a.n0 = (int)in.n[0];
+  a.n1 = (int)in.n[1];
+  a.o0 = (int)ref.n[0];
+  a.o1 = (int)ref.n[1];
+  a.f0 = (int)out_first[0];
+  a.f1 = (int)out_first[1];
+  a.wf0 = wfirst[0];
+  a.wf1 = wfirst[1];
+  a.wl0 = wlen[0];
+  a.wl1 = wlen[1];
+  a.isfill = isfill;
+  a.fillkey = fill_key32(in.dtype, fill);
+  a.in_plane = in.n[0] * in.n[1];
+  a.out_plane = ref.n[0] * ref.n[1];
+  a.nplanes = (int)(ref.n[2] * ref.n[3]);
+  const int iw = MM_TX + a.wl0 - 1, ih = MM_TY + a.wl1 - 1;
+  const size_t lds = sizeof(uint32_t) * ((size_t)ih * (iw | 1) + 2 * (size_t)ih * MM_TX);
+  dim3 grid((a.o0 + MM_TX - 1) / MM_TX, (a.o1 + MM_TY - 1) / MM_TY, a.nplanes < 65535 ? a.nplanes : 65535);
+  switch (in.dtype) {
+  case IFM_F32: mm_dispatch<IFM_F32>(a, grid, lds, st); break;
+  case IFM_BF16: mm_dispatch<IFM_BF16>(a, grid, lds, st); break;
+  case IFM_F16: mm_dispatch<IFM_F16>(a, grid, lds, st); break;
+  default: mm_dispatch<IFM_U8>(a, grid, lds, st); break;  // U8 and N0f8: the code is the key
+  }
+  *err = hipGetLastError();
+  return true;
+}
+
 size_t minmax_workspace(const Dense4 &in, const int64_t *osize) {
   // two ping-pong key buffers per extreme, sized for the largest intermediate
   size_t ks = in.dtype == IFM_F64 ? 8 : 4;
@@ -506,6 +888,8 @@ hipError_t launch_minmax(const Dense4 &in, Dense4 *omin, Dense4 *omax, const int
                          hipStream_t st) {
   Dense4 &ref = omin ? *omin : *omax;
   if (ref.count() == 0) return hipSuccess;
+  hipError_t e2d;
+  if (minmax2d(in, omin, omax, out_first, wfirst, wlen, isfill, fill, st, &e2d)) return e2d;
   bool k64 = in.dtype == IFM_F64;
   size_t ks = k64 ? 8 : 4;
   int64_t osz[MAXD];

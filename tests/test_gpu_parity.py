@@ -460,6 +460,37 @@ def test_minmax_bit_exact(w):
     np.testing.assert_array_equal(IF.mapwindow(IF.maximum, f, (w, w)), mx.astype(np.float32))
 
 
+@pytest.mark.parametrize("dtype", ["u8", "n0f8", "f32", "bf16"])
+def test_minmax_tiled_many_tiles(dtype):
+    """The one-pass 2-D min/max (several tiles and planes, square and
+    non-square windows, compile-time and run-time window paths, NaN)."""
+    rng = np.random.default_rng(27)
+    codes = rng.integers(0, 256, (301, 170, 3)).astype(np.uint8)
+    if dtype in ("u8", "n0f8"):
+        img = IF.n0f8(codes) if dtype == "n0f8" else codes
+        x64 = codes.astype(np.float64)
+    else:
+        f = rng.standard_normal(codes.shape).astype(np.float32)
+        f[5, 7, 0] = np.nan
+        f[200, 100, 2] = -0.0
+        img = IF.bf16(f) if dtype == "bf16" else f
+        x64 = (IF.bf16_to_float32(img) if dtype == "bf16" else f).astype(np.float64)
+    for win in [(3, 3), (9, 9), (5, 7), (1, 11), (33, 3)]:
+        wf = [-(w // 2) for w in win] + [0]
+        for b in ["replicate", "reflect", IF.Fill(0)]:
+            mn, mx = O.minmax(x64, wf, list(win) + [1], obord(b))
+            gmn = IF.mapwindow(IF.minimum, img, win + (1,), border=b)
+            gmx = IF.mapwindow(IF.maximum, img, win + (1,), border=b)
+            if dtype == "bf16":
+                gmn, gmx = IF.bf16_to_float32(gmn), IF.bf16_to_float32(gmx)
+            gmn = np.asarray(gmn).astype(np.float64)  # N0f8 results are their uint8 codes
+            gmx = np.asarray(gmx).astype(np.float64)
+            np.testing.assert_array_equal(gmn, mn, err_msg=str((dtype, win, b)))
+            np.testing.assert_array_equal(gmx, mx, err_msg=str((dtype, win, b)))
+            if dtype == "f32":
+                assert np.signbit(gmn[200, 100, 2]) == np.signbit(mn[200, 100, 2])
+
+
 def test_extrema_lemire_semantics():
     import json, os
     g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_fixtures.json")))

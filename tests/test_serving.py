@@ -177,7 +177,8 @@ def test_batch_rejects_bad_requests():
     arr, k2 = serving.requests(mix, bufs)
     L, ctx = _lib.load(), _lib.ctx()
     bad = (_lib.IfmRequest * 1)(arr[0])
-    bad[0].out.size[0] += 1
+    bad[0].out.size[0] += 1                      # rejected before any launch
+    bad[0].out.stride[1] = bad[0].out.size[0]
     with pytest.raises(IF.DimensionMismatch):
         _lib.check(L.ifm_batch_run(ctx, bad, 1))
     bad = (_lib.IfmRequest * 1)(arr[2])

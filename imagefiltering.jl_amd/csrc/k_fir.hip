@@ -224,6 +224,156 @@ __global__ void __launch_bounds__(256) k_sep2d(Sep2dArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// sep3d: the three-factor cascade on axes 0, 1 and 2 in one pass.  A
+// workgroup owns a 64 x 32 column of outputs in (x, y) and up to S3_NZ output
+// planes; it marches over the S3_NZ + K2 - 1 input planes they need, filters
+// each along axes 0 and 1 (sep2d_compute, border map on load) and adds it into
+// the K2 pending output planes kept in registers, writing a plane as soon as
+// its last input plane has arrived.  HBM sees the input once (plus halos) and
+// the output once; there is no temporary.  Accumulation along axis 2 runs in
+// tap order, like the cascade's third pass.
+// ---------------------------------------------------------------------------
+constexpr int S3_NZ = 32;
+constexpr int S3_MAXK2 = 17;
+
+struct Sep3dArgs {
+  const void *in;
+  void *out;
+  int n0, n1, n2;       // input extents (axes 0..2)
+  int o0, o1, o2;       // output extents
+  int f0, f1, f2, f3;   // out_first
+  int64_t in_plane, out_plane;  // elements per (x, y) plane
+  int n3;               // planes along axis 3 (output)
+  int nzc;              // z chunks
+  int k0, k1, first0, first1, first2;
+  int style, dt_in, dt_out;
+  float fill;           // the Fill value
+  float fill01;         // the Fill value filtered by the axis-0 and axis-1 factors
+  float taps0[S2_MAXK];
+  float taps1[S2_MAXK];
+  float taps2[S3_MAXK2];
+};
+
+template <int K2>
+__global__ void __launch_bounds__(256) k_sep3d(Sep3dArgs a) {
+  extern __shared__ float smem[];
+  const int zc = blockIdx.z % a.nzc, p3 = blockIdx.z / a.nzc;
+  const int oz0 = zc * S3_NZ, nz = min(S3_NZ, a.o2 - oz0);
+  Sep2dTile t;
+  t.in = a.in;
+  t.out = a.out;
+  t.n0 = a.n0;
+  t.n1 = a.n1;
+  t.o0 = a.o0;
+  t.o1 = a.o1;
+  t.k0 = a.k0;
+  t.k1 = a.k1;
+  t.first0 = a.first0;
+  t.first1 = a.first1;
+  t.style = a.style;
+  t.dt_in = a.dt_in;
+  t.dt_out = a.dt_out;
+  t.fill = a.fill;
+  t.ox0 = blockIdx.x * S2_TX;
+  t.oy0 = blockIdx.y * S2_TY;
+  t.x0 = a.f0 + t.ox0;
+  t.y0 = a.f1 + t.oy0;
+  // When input plane q arrives, acc[s] holds output plane q - (K2 - 1) + s:
+  // plane q adds taps2[j] to output q - j, i.e. slot K2 - 1 - j; afterwards
+  // slot 0 is complete and the window shifts by one (static register indices).
+  float acc[K2][8];
+#pragma unroll
+  for (int s = 0; s < K2; ++s)
+#pragma unroll
+    for (int r = 0; r < 8; ++r) acc[s][r] = 0.f;
+  const int nq = nz + K2 - 1;
+  for (int q = 0; q < nq; ++q) {
+    const int sz = bmap(a.style, a.f2 + oz0 + a.first2 + q, a.n2);
+    float sl[8];
+    if (sz < 0) {
+#pragma unroll
+      for (int r = 0; r < 8; ++r) sl[r] = a.fill01;
+    } else {
+      t.ibase = ((int64_t)(p3 + a.f3) * a.n2 + sz) * a.in_plane;
+      sep2d_compute(t, a.taps0, a.taps1, smem, sl);
+      __syncthreads();  // the next plane reuses the LDS tiles
+    }
+#pragma unroll
+    for (int j = 0; j < K2; ++j)
+#pragma unroll
+      for (int r = 0; r < 8; ++r) acc[K2 - 1 - j][r] = fmaf(a.taps2[j], sl[r], acc[K2 - 1 - j][r]);
+    const int od = q - (K2 - 1);  // complete (slots of outputs below 0 are dropped)
+    if (od >= 0) {
+      t.obase = ((int64_t)p3 * a.o2 + oz0 + od) * a.out_plane;
+      sep2d_store(t, acc[0]);
+    }
+#pragma unroll
+    for (int s = 0; s + 1 < K2; ++s)
+#pragma unroll
+      for (int r = 0; r < 8; ++r) acc[s][r] = acc[s + 1][r];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) acc[K2 - 1][r] = 0.f;
+  }
+}
+
+bool sep3d_supported(int k0, int k1, int k2) { return k0 <= S2_MAXK && k1 <= S2_MAXK && k2 <= S3_MAXK2 && (k2 & 1); }
+
+hipError_t launch_sep3d(const Dense4 &in, Dense4 &out, const int64_t *out_first, const FirPass &p0, const FirPass &p1,
+                        const FirPass &p2, int style, float fill, hipStream_t st) {
+  if (out.count() == 0) return hipSuccess;
+  Sep3dArgs a;
+  a.in = in.data;
+  a.out = out.data;
+  a.n0 = (int)in.n[0];
+  a.n1 = (int)in.n[1];
+  a.n2 = (int)in.n[2];
+  a.o0 = (int)out.n[0];
+  a.o1 = (int)out.n[1];
+  a.o2 = (int)out.n[2];
+  a.f0 = (int)out_first[0];
+  a.f1 = (int)out_first[1];
+  a.f2 = (int)out_first[2];
+  a.f3 = (int)out_first[3];
+  a.in_plane = in.n[0] * in.n[1];
+  a.out_plane = out.n[0] * out.n[1];
+  a.n3 = (int)out.n[3];
+  a.nzc = (a.o2 + S3_NZ - 1) / S3_NZ;
+  a.k0 = p0.len;
+  a.k1 = p1.len;
+  a.first0 = p0.first;
+  a.first1 = p1.first;
+  a.first2 = p2.first;
+  a.style = style;
+  a.dt_in = in.dtype;
+  a.dt_out = out.dtype;
+  a.fill = fill;
+  float s0 = 0.f, s1 = 0.f;
+  for (int t = 0; t < p0.len; ++t) s0 += p0.taps[t];
+  for (int t = 0; t < p1.len; ++t) s1 += p1.taps[t];
+  a.fill01 = fill * s0 * s1;
+  for (int t = 0; t < p0.len; ++t) a.taps0[t] = p0.taps[t];
+  for (int t = 0; t < p1.len; ++t) a.taps1[t] = p1.taps[t];
+  for (int t = 0; t < p2.len; ++t) a.taps2[t] = p2.taps[t];
+  const size_t lds = sizeof(float) * sep2d_lds_floats(a.k0, a.k1);
+  const int64_t gz = (int64_t)a.nzc * a.n3;
+  if (gz > 65535) return hipErrorInvalidValue;
+  dim3 grid((a.o0 + S2_TX - 1) / S2_TX, (a.o1 + S2_TY - 1) / S2_TY, (unsigned)gz);
+  switch (p2.len) {
+  case 1: hipLaunchKernelGGL(k_sep3d<1>, grid, dim3(256), lds, st, a); break;
+  case 3: hipLaunchKernelGGL(k_sep3d<3>, grid, dim3(256), lds, st, a); break;
+  case 5: hipLaunchKernelGGL(k_sep3d<5>, grid, dim3(256), lds, st, a); break;
+  case 7: hipLaunchKernelGGL(k_sep3d<7>, grid, dim3(256), lds, st, a); break;
+  case 9: hipLaunchKernelGGL(k_sep3d<9>, grid, dim3(256), lds, st, a); break;
+  case 11: hipLaunchKernelGGL(k_sep3d<11>, grid, dim3(256), lds, st, a); break;
+  case 13: hipLaunchKernelGGL(k_sep3d<13>, grid, dim3(256), lds, st, a); break;
+  case 15: hipLaunchKernelGGL(k_sep3d<15>, grid, dim3(256), lds, st, a); break;
+  case 17: hipLaunchKernelGGL(k_sep3d<17>, grid, dim3(256), lds, st, a); break;
+  default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 // Batched serving: tile g of the launch is tile g - tile_first[r] of request
 // r = tile_req[g] (row-major over its 64 x 32 tiles).
 __global__ void __launch_bounds__(256) k_sep_batch(const SepReq *__restrict__ reqs, const int *__restrict__ tile_req,

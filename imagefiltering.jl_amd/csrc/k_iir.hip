@@ -23,38 +23,68 @@ struct IirArgs {
   double fill;
 };
 
-// Line accessor: element i of line `base` with stride `st`.
+// The recursions are sequential along a line, so the inputs of the next
+// IC steps are loaded into registers before those steps run (a step never
+// waits for its own load).  Reading ahead is safe in place: the forward pass
+// only writes positions it has already read, the backward pass likewise.
+constexpr int IC = 16;
+
+// Source element as FP64; DT < 0: element type read at run time.
+template <int DT>
+__device__ __forceinline__ double iir_ld(const IirArgs &p, int64_t i) {
+  if constexpr (DT == IFM_F32) return (double)((const float *)p.src)[i];
+  else if constexpr (DT == IFM_F64) return ((const double *)p.src)[i];
+  else return load_d(p.src, i, (int)p.dt_src);
+}
+
+// Line accessor: element i of line `base` with stride `st` (ST1: st == 1, so
+// a chunk is contiguous and the loads vectorise).
+template <bool ST1, int DT>
 __global__ void __launch_bounds__(256) k_iir_strided(IirArgs p) {
   const int64_t line = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (line >= p.lines) return;
   const int n = p.n[p.axis];
-  const int64_t st = p.stride;
+  const int64_t st = ST1 ? 1 : p.stride;
   const int64_t pre = line % st, post = line / st;
   const int64_t base = pre + post * st * n;
   const double a1 = p.a[0], a2 = p.a[1], a3 = p.a[2];
   const double b1 = p.b[0], b2 = p.b[1], b3 = p.b[2];
   const double asum = a1 + a2 + a3, bsum = b1 + b2 + b3;
-  const double iminus = p.isfill ? p.fill : load_d(p.src, base, p.dt_src);
-  const double iplus = p.isfill ? p.fill : load_d(p.src, base + (int64_t)(n - 1) * st, p.dt_src);
+  const double iminus = p.isfill ? p.fill : iir_ld<DT>(p, base);
+  const double iplus = p.isfill ? p.fill : iir_ld<DT>(p, base + (int64_t)(n - 1) * st);
   // leftborder!
   const double uminus = iminus / (1 - asum);
-  double y0 = load_d(p.src, base, p.dt_src) + a1 * uminus + a2 * uminus + a3 * uminus;
+  double y0 = iir_ld<DT>(p, base) + a1 * uminus + a2 * uminus + a3 * uminus;
   p.dst[base] = y0;
-  double y1 = load_d(p.src, base + st, p.dt_src) + a1 * y0 + a2 * uminus + a3 * uminus;
+  double y1 = iir_ld<DT>(p, base + st) + a1 * y0 + a2 * uminus + a3 * uminus;
   p.dst[base + st] = y1;
-  double y2 = load_d(p.src, base + 2 * st, p.dt_src) + a1 * y1 + a2 * y0 + a3 * uminus;
+  double y2 = iir_ld<DT>(p, base + 2 * st) + a1 * y1 + a2 * y0 + a3 * uminus;
   p.dst[base + 2 * st] = y2;
   // forward, omitting the final point
   double ym3 = y0, ym2 = y1, ym1 = y2;
-  for (int i = 3; i <= n - 2; ++i) {
-    double y = load_d(p.src, base + (int64_t)i * st, p.dt_src) + a1 * ym1 + a2 * ym2 + a3 * ym3;
+  int i = 3;
+  for (; i + IC - 1 <= n - 2; i += IC) {
+    double x[IC];
+#pragma unroll
+    for (int j = 0; j < IC; ++j) x[j] = iir_ld<DT>(p, base + (int64_t)(i + j) * st);
+#pragma unroll
+    for (int j = 0; j < IC; ++j) {
+      const double y = x[j] + a1 * ym1 + a2 * ym2 + a3 * ym3;
+      p.dst[base + (int64_t)(i + j) * st] = y;
+      ym3 = ym2;
+      ym2 = ym1;
+      ym1 = y;
+    }
+  }
+  for (; i <= n - 2; ++i) {
+    double y = iir_ld<DT>(p, base + (int64_t)i * st) + a1 * ym1 + a2 * ym2 + a3 * ym3;
     p.dst[base + (int64_t)i * st] = y;
     ym3 = ym2;
     ym2 = ym1;
     ym1 = y;
   }
   // rightborder!: the last forward value, then v at and beyond the edge
-  const double ylast = load_d(p.src, base + (int64_t)(n - 1) * st, p.dt_src) + a1 * ym1 + a2 * ym2 + a3 * ym3;
+  const double ylast = iir_ld<DT>(p, base + (int64_t)(n - 1) * st) + a1 * ym1 + a2 * ym2 + a3 * ym3;
   const double uplus = iplus / (1 - asum);
   const double vplus = uplus / (1 - bsum);
   const double d0 = ylast - uplus, d1 = ym1 - uplus, d2 = ym2 - uplus;
@@ -72,9 +102,23 @@ __global__ void __launch_bounds__(256) k_iir_strided(IirArgs p) {
   p.dst[base + (int64_t)(n - 3) * st] = z0 * sc;
   // backward
   double zp1 = z0, zp2 = z1, zp3 = z2;
-  for (int i = n - 4; i >= 0; --i) {
-    double z = p.dst[base + (int64_t)i * st] + b1 * zp1 + b2 * zp2 + b3 * zp3;
-    p.dst[base + (int64_t)i * st] = z * sc;
+  int k = n - 4;
+  for (; k - (IC - 1) >= 0; k -= IC) {
+    double y[IC];
+#pragma unroll
+    for (int j = 0; j < IC; ++j) y[j] = p.dst[base + (int64_t)(k - j) * st];
+#pragma unroll
+    for (int j = 0; j < IC; ++j) {
+      const double z = y[j] + b1 * zp1 + b2 * zp2 + b3 * zp3;
+      p.dst[base + (int64_t)(k - j) * st] = z * sc;
+      zp3 = zp2;
+      zp2 = zp1;
+      zp1 = z;
+    }
+  }
+  for (; k >= 0; --k) {
+    double z = p.dst[base + (int64_t)k * st] + b1 * zp1 + b2 * zp2 + b3 * zp3;
+    p.dst[base + (int64_t)k * st] = z * sc;
     zp3 = zp2;
     zp2 = zp1;
     zp1 = z;
@@ -105,7 +149,17 @@ hipError_t launch_iir(const Dense4 &in, Dense4 &out, int axis, const IirCoef &c,
   p.fill = fill;
   if (p.lines == 0) return hipSuccess;
   int64_t blocks = (p.lines + 255) / 256;
-  hipLaunchKernelGGL(k_iir_strided, dim3((unsigned)blocks), dim3(256), 0, st, p);
+  const bool s1 = stride == 1;
+  if (in.dtype == IFM_F32) {
+    if (s1) hipLaunchKernelGGL((k_iir_strided<true, IFM_F32>), dim3((unsigned)blocks), dim3(256), 0, st, p);
+    else hipLaunchKernelGGL((k_iir_strided<false, IFM_F32>), dim3((unsigned)blocks), dim3(256), 0, st, p);
+  } else if (in.dtype == IFM_F64) {
+    if (s1) hipLaunchKernelGGL((k_iir_strided<true, IFM_F64>), dim3((unsigned)blocks), dim3(256), 0, st, p);
+    else hipLaunchKernelGGL((k_iir_strided<false, IFM_F64>), dim3((unsigned)blocks), dim3(256), 0, st, p);
+  } else {
+    if (s1) hipLaunchKernelGGL((k_iir_strided<true, -1>), dim3((unsigned)blocks), dim3(256), 0, st, p);
+    else hipLaunchKernelGGL((k_iir_strided<false, -1>), dim3((unsigned)blocks), dim3(256), 0, st, p);
+  }
   return hipGetLastError();
 }
 

@@ -61,10 +61,11 @@ __device__ __forceinline__ void sep2d_stage(const Sep2dTile &a, float *s_in, int
   const bool colok = c < in_w;
   const int sx = bmap(a.style, a.x0 + a.first0 + (colok ? c : in_w - 1), a.n0);
   const int nrow = (in_h - rh + 1) >> 1;                // rows of this half
-  for (int j0 = 0; j0 < nrow; j0 += 8) {
-    float v[8];
+  constexpr int NV = 16;                                // loads in flight per thread
+  for (int j0 = 0; j0 < nrow; j0 += NV) {
+    float v[NV];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
+    for (int j = 0; j < NV; ++j) {
       const int jj = j0 + j < nrow ? j0 + j : nrow - 1;  // clamped: every load is in bounds
       const int sy = bmap(a.style, a.y0 + a.first1 + rh + 2 * jj, a.n1);
       const bool fill = sy < 0 || sx < 0;
@@ -73,14 +74,40 @@ __device__ __forceinline__ void sep2d_stage(const Sep2dTile &a, float *s_in, int
     }
     if (colok) {
 #pragma unroll
-      for (int j = 0; j < 8; ++j)
+      for (int j = 0; j < NV; ++j)
         if (j0 + j < nrow) s_in[(rh + 2 * (j0 + j)) * in_wp + c] = v[j];
+    }
+  }
+}
+
+// Stages and filters one tile; on return thread t holds the outputs of column
+// t % 64, rows 8 (t / 64) .. +7 of the tile in acc.  The caller owns the
+// barrier before LDS is reused.
+__device__ __forceinline__ void sep2d_compute(const Sep2dTile &a, const float *taps0, const float *taps1,
+                                              float *smem, float (&acc)[8]);
+
+__device__ __forceinline__ void sep2d_store(const Sep2dTile &a, const float (&acc)[8]) {
+  const int tid = threadIdx.x;
+  const int col = tid & 63, r0 = (tid >> 6) * 8;
+  const int ox = a.ox0 + col;
+  if (ox < a.o0) {
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+      const int oy = a.oy0 + r0 + o;
+      if (oy < a.o1) store_f(a.out, a.obase + (int64_t)oy * a.o0 + ox, a.dt_out, acc[o]);
     }
   }
 }
 
 __device__ __forceinline__ void sep2d_tile(const Sep2dTile &a, const float *taps0, const float *taps1,
                                            float *smem) {
+  float acc[8];
+  sep2d_compute(a, taps0, taps1, smem, acc);
+  sep2d_store(a, acc);
+}
+
+__device__ __forceinline__ void sep2d_compute(const Sep2dTile &a, const float *taps0, const float *taps1,
+                                              float *smem, float (&acc)[8]) {
   const int in_w = S2_TX + a.k0 - 1;
   const int in_h = S2_TY + a.k1 - 1;
   const int in_wp = in_w | 1;
@@ -100,41 +127,35 @@ __device__ __forceinline__ void sep2d_tile(const Sep2dTile &a, const float *taps
   for (int item = tid; item < in_h * 16; item += 256) {
     const int row = item >> 4, cq = (item & 15) * 4;
     const float *src = s_in + row * in_wp + cq;
-    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    float q[4] = {0.f, 0.f, 0.f, 0.f};
     float w[4] = {src[0], src[1], src[2], 0.f};
+    // unrolled so that several LDS reads are in flight
+#pragma unroll 8
     for (int t = 0; t < a.k0; ++t) {
       w[3] = src[t + 3];
       const float k = taps0[t];
 #pragma unroll
-      for (int o = 0; o < 4; ++o) acc[o] = fmaf(k, w[o], acc[o]);
+      for (int o = 0; o < 4; ++o) q[o] = fmaf(k, w[o], q[o]);
       w[0] = w[1];
       w[1] = w[2];
       w[2] = w[3];
     }
-    float4 r = make_float4(acc[0], acc[1], acc[2], acc[3]);
+    float4 r = make_float4(q[0], q[1], q[2], q[3]);
     *(float4 *)(s_mid + row * S2_TX + cq) = r;
   }
   __syncthreads();
   // Axis-1 factor: each thread owns 8 consecutive rows of one column.
   const int col = tid & 63, r0 = (tid >> 6) * 8;
-  float acc[8];
 #pragma unroll
   for (int o = 0; o < 8; ++o) acc[o] = 0.f;
   const float *src = s_mid + r0 * S2_TX + col;
+#pragma unroll 4
   for (int u = 0; u < a.k1 + 7; ++u) {
     const float v = src[u * S2_TX];
 #pragma unroll
     for (int o = 0; o < 8; ++o) {
       const int j = u - o;
       if (j >= 0 && j < a.k1) acc[o] = fmaf(taps1[j], v, acc[o]);
-    }
-  }
-  const int ox = a.ox0 + col;
-  if (ox < a.o0) {
-#pragma unroll
-    for (int o = 0; o < 8; ++o) {
-      const int oy = a.oy0 + r0 + o;
-      if (oy < a.o1) store_f(a.out, a.obase + (int64_t)oy * a.o0 + ox, a.dt_out, acc[o]);
     }
   }
 }

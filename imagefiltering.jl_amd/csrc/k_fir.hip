@@ -255,7 +255,9 @@ struct Sep3dArgs {
   float taps2[S3_MAXK2];
 };
 
-template <int K2>
+constexpr int S3_NV = 24;  // staged rows per thread: in_h <= 48, i.e. k1 <= 17
+
+template <int K2, int DT>
 __global__ void __launch_bounds__(256) k_sep3d(Sep3dArgs a) {
   extern __shared__ float smem[];
   const int zc = blockIdx.z % a.nzc, p3 = blockIdx.z / a.nzc;
@@ -287,18 +289,36 @@ __global__ void __launch_bounds__(256) k_sep3d(Sep3dArgs a) {
   for (int s = 0; s < K2; ++s)
 #pragma unroll
     for (int r = 0; r < 8; ++r) acc[s][r] = 0.f;
+  // Software pipeline over input planes: the loads of plane q + 1 are issued
+  // (raw bits, into registers) before plane q is filtered, and committed to
+  // LDS at the top of the next step.
+  const int in_w = S2_TX + a.k0 - 1, in_h = S2_TY + a.k1 - 1, in_wp = in_w | 1;
+  const StageLane ln = stage_lane(t, in_w, in_h);  // nrow <= S3_NV (k1 <= 17)
+  uint32_t v[S3_NV];
   const int nq = nz + K2 - 1;
+  auto plane_of = [&](int q) { return bmap(a.style, a.f2 + oz0 + a.first2 + q, a.n2); };
+  int sz = plane_of(0);
+  if (sz >= 0) {
+    t.ibase = ((int64_t)(p3 + a.f3) * a.n2 + sz) * a.in_plane;
+    sep2d_issue<DT, S3_NV>(t, ln, 0, v);
+  }
   for (int q = 0; q < nq; ++q) {
-    const int sz = bmap(a.style, a.f2 + oz0 + a.first2 + q, a.n2);
     float sl[8];
+    if (sz >= 0) sep2d_commit<DT, S3_NV>(t, ln, smem, in_wp, 0, v);
+    __syncthreads();  // plane q staged
+    const int szn = q + 1 < nq ? plane_of(q + 1) : -1;
+    if (szn >= 0) {
+      t.ibase = ((int64_t)(p3 + a.f3) * a.n2 + szn) * a.in_plane;
+      sep2d_issue<DT, S3_NV>(t, ln, 0, v);
+    }
     if (sz < 0) {
 #pragma unroll
       for (int r = 0; r < 8; ++r) sl[r] = a.fill01;
     } else {
-      t.ibase = ((int64_t)(p3 + a.f3) * a.n2 + sz) * a.in_plane;
-      sep2d_compute(t, a.taps0, a.taps1, smem, sl);
-      __syncthreads();  // the next plane reuses the LDS tiles
+      sep2d_filter(t, a.taps0, a.taps1, smem, sl);
     }
+    __syncthreads();  // every wave is done with plane q's LDS tiles
+    sz = szn;
 #pragma unroll
     for (int j = 0; j < K2; ++j)
 #pragma unroll
@@ -317,7 +337,18 @@ __global__ void __launch_bounds__(256) k_sep3d(Sep3dArgs a) {
   }
 }
 
-bool sep3d_supported(int k0, int k1, int k2) { return k0 <= S2_MAXK && k1 <= S2_MAXK && k2 <= S3_MAXK2 && (k2 & 1); }
+bool sep3d_supported(int k0, int k1, int k2, int dt_in) {
+  return k0 <= S2_MAXK && S2_TY + k1 - 1 <= 2 * S3_NV && k2 <= S3_MAXK2 && (k2 & 1) &&
+         (dt_in == IFM_F32 || dt_in == IFM_F64);
+}
+
+template <int K2>
+static void sep3d_go(const Sep3dArgs &a, bool f64, dim3 grid, size_t lds, hipStream_t st) {
+  if (f64)
+    hipLaunchKernelGGL((k_sep3d<K2, IFM_F64>), grid, dim3(256), lds, st, a);
+  else
+    hipLaunchKernelGGL((k_sep3d<K2, IFM_F32>), grid, dim3(256), lds, st, a);
+}
 
 hipError_t launch_sep3d(const Dense4 &in, Dense4 &out, const int64_t *out_first, const FirPass &p0, const FirPass &p1,
                         const FirPass &p2, int style, float fill, hipStream_t st) {
@@ -359,16 +390,17 @@ hipError_t launch_sep3d(const Dense4 &in, Dense4 &out, const int64_t *out_first,
   const int64_t gz = (int64_t)a.nzc * a.n3;
   if (gz > 65535) return hipErrorInvalidValue;
   dim3 grid((a.o0 + S2_TX - 1) / S2_TX, (a.o1 + S2_TY - 1) / S2_TY, (unsigned)gz);
+  const bool f64 = in.dtype == IFM_F64;
   switch (p2.len) {
-  case 1: hipLaunchKernelGGL(k_sep3d<1>, grid, dim3(256), lds, st, a); break;
-  case 3: hipLaunchKernelGGL(k_sep3d<3>, grid, dim3(256), lds, st, a); break;
-  case 5: hipLaunchKernelGGL(k_sep3d<5>, grid, dim3(256), lds, st, a); break;
-  case 7: hipLaunchKernelGGL(k_sep3d<7>, grid, dim3(256), lds, st, a); break;
-  case 9: hipLaunchKernelGGL(k_sep3d<9>, grid, dim3(256), lds, st, a); break;
-  case 11: hipLaunchKernelGGL(k_sep3d<11>, grid, dim3(256), lds, st, a); break;
-  case 13: hipLaunchKernelGGL(k_sep3d<13>, grid, dim3(256), lds, st, a); break;
-  case 15: hipLaunchKernelGGL(k_sep3d<15>, grid, dim3(256), lds, st, a); break;
-  case 17: hipLaunchKernelGGL(k_sep3d<17>, grid, dim3(256), lds, st, a); break;
+  case 1: sep3d_go<1>(a, f64, grid, lds, st); break;
+  case 3: sep3d_go<3>(a, f64, grid, lds, st); break;
+  case 5: sep3d_go<5>(a, f64, grid, lds, st); break;
+  case 7: sep3d_go<7>(a, f64, grid, lds, st); break;
+  case 9: sep3d_go<9>(a, f64, grid, lds, st); break;
+  case 11: sep3d_go<11>(a, f64, grid, lds, st); break;
+  case 13: sep3d_go<13>(a, f64, grid, lds, st); break;
+  case 15: sep3d_go<15>(a, f64, grid, lds, st); break;
+  case 17: sep3d_go<17>(a, f64, grid, lds, st); break;
   default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -10,8 +10,9 @@ namespace ifm {
 // src/imfilter.jl:372-457 with the factor loop of :671-739).  One workgroup of
 // 256 threads computes a 64 x 32 output tile:
 //   1. the (32 + k1 - 1) x (64 + k0 - 1) input tile is loaded once into LDS,
-//      border map on load; each thread owns one column and every other row,
-//      and issues 8 rows of loads before it writes any of them;
+//      border map on load; each thread owns one column and every other row.
+//      Loads keep the raw element bits and are issued in groups before any
+//      is used (issue / commit), so several are in flight per thread;
 //   2. the axis-0 factor runs from registers: a thread slides over k0 + 3
 //      LDS values to produce 4 consecutive outputs of one row (mid tile);
 //   3. the axis-1 factor likewise produces 8 consecutive rows of one column.
@@ -43,48 +44,77 @@ __host__ __device__ inline size_t sep2d_lds_floats(int k0, int k1) {
   return (size_t)sep2d_mid_offset(k0, k1) + (size_t)(S2_TY + k1 - 1) * S2_TX;
 }
 
+// Raw element bits (F64 is narrowed at once) and their value: a load is not
+// waited for until the value is formed.
 template <int DT>
-__device__ __forceinline__ float load_as(const void *p, int64_t i) {
-  if constexpr (DT == IFM_U8) return (float)((const uint8_t *)p)[i];
-  else if constexpr (DT == IFM_N0F8) return (float)((const uint8_t *)p)[i] * (1.0f / 255.0f);
-  else if constexpr (DT == IFM_F32) return ((const float *)p)[i];
-  else if constexpr (DT == IFM_F64) return (float)((const double *)p)[i];
-  else if constexpr (DT == IFM_BF16) return bf16_to_f32(((const uint16_t *)p)[i]);
-  else return __half2float(((const __half *)p)[i]);
+__device__ __forceinline__ uint32_t raw_ld(const void *p, int64_t i) {
+  if constexpr (DT == IFM_F32) return ((const uint32_t *)p)[i];
+  else if constexpr (DT == IFM_BF16 || DT == IFM_F16) return ((const uint16_t *)p)[i];
+  else if constexpr (DT == IFM_U8 || DT == IFM_N0F8) return ((const uint8_t *)p)[i];
+  else return __float_as_uint((float)((const double *)p)[i]);
+}
+template <int DT>
+__device__ __forceinline__ float raw_val(uint32_t r) {
+  if constexpr (DT == IFM_F32 || DT == IFM_F64) return __uint_as_float(r);
+  else if constexpr (DT == IFM_BF16) return __uint_as_float(r << 16);
+  else if constexpr (DT == IFM_F16) return (float)__builtin_bit_cast(_Float16, (uint16_t)r);
+  else if constexpr (DT == IFM_U8) return (float)r;
+  else return (float)r * (1.0f / 255.0f);
 }
 
-template <int DT>
-__device__ __forceinline__ void sep2d_stage(const Sep2dTile &a, float *s_in, int in_w, int in_h, int in_wp) {
-  const int tid = threadIdx.x;
-  const int c = tid & 127;                               // the thread's column
-  const int rh = __builtin_amdgcn_readfirstlane(tid >> 7);  // rows rh, rh + 2, ...
-  const bool colok = c < in_w;
-  const int sx = bmap(a.style, a.x0 + a.first0 + (colok ? c : in_w - 1), a.n0);
-  const int nrow = (in_h - rh + 1) >> 1;                // rows of this half
-  constexpr int NV = 16;                                // loads in flight per thread
-  for (int j0 = 0; j0 < nrow; j0 += NV) {
-    float v[NV];
+// Thread's staging coordinates: column c (of in_w), rows rh, rh + 2, ...
+struct StageLane {
+  int c, rh, nrow, sx;
+  bool colok;
+};
+__device__ __forceinline__ StageLane stage_lane(const Sep2dTile &a, int in_w, int in_h) {
+  StageLane s;
+  s.c = threadIdx.x & 127;
+  s.rh = __builtin_amdgcn_readfirstlane(threadIdx.x >> 7);
+  s.colok = s.c < in_w;
+  s.sx = bmap(a.style, a.x0 + a.first0 + (s.colok ? s.c : in_w - 1), a.n0);
+  s.nrow = (in_h - s.rh + 1) >> 1;
+  return s;
+}
+
+// Loads of rows j0 .. j0+NV-1 of the thread (Fill positions load element 0).
+template <int DT, int NV>
+__device__ __forceinline__ void sep2d_issue(const Sep2dTile &a, const StageLane &s, int j0, uint32_t (&v)[NV]) {
+  const int sxc = s.sx < 0 ? 0 : s.sx;
 #pragma unroll
-    for (int j = 0; j < NV; ++j) {
-      const int jj = j0 + j < nrow ? j0 + j : nrow - 1;  // clamped: every load is in bounds
-      const int sy = bmap(a.style, a.y0 + a.first1 + rh + 2 * jj, a.n1);
-      const bool fill = sy < 0 || sx < 0;
-      const float x = load_as<DT>(a.in, a.ibase + (fill ? 0 : (int64_t)sy * a.n0 + sx));
-      v[j] = fill ? a.fill : x;
-    }
-    if (colok) {
-#pragma unroll
-      for (int j = 0; j < NV; ++j)
-        if (j0 + j < nrow) s_in[(rh + 2 * (j0 + j)) * in_wp + c] = v[j];
+  for (int j = 0; j < NV; ++j) {
+    const int jj = j0 + j;
+    if (jj < s.nrow) {  // wave-uniform
+      const int sy = bmap(a.style, a.y0 + a.first1 + s.rh + 2 * jj, a.n1);
+      v[j] = raw_ld<DT>(a.in, a.ibase + (sy < 0 ? 0 : (int64_t)sy * a.n0 + sxc));
     }
   }
 }
 
-// Stages and filters one tile; on return thread t holds the outputs of column
-// t % 64, rows 8 (t / 64) .. +7 of the tile in acc.  The caller owns the
-// barrier before LDS is reused.
-__device__ __forceinline__ void sep2d_compute(const Sep2dTile &a, const float *taps0, const float *taps1,
-                                              float *smem, float (&acc)[8]);
+template <int DT, int NV>
+__device__ __forceinline__ void sep2d_commit(const Sep2dTile &a, const StageLane &s, float *s_in, int in_wp, int j0,
+                                             const uint32_t (&v)[NV]) {
+  if (!s.colok) return;
+#pragma unroll
+  for (int j = 0; j < NV; ++j) {
+    const int jj = j0 + j;
+    if (jj < s.nrow) {
+      const int sy = bmap(a.style, a.y0 + a.first1 + s.rh + 2 * jj, a.n1);
+      s_in[(s.rh + 2 * jj) * in_wp + s.c] = (sy < 0 || s.sx < 0) ? a.fill : raw_val<DT>(v[j]);
+    }
+  }
+}
+
+template <int DT>
+__device__ __forceinline__ void sep2d_stage(const Sep2dTile &a, float *s_in, int in_w, int in_h, int in_wp) {
+  const StageLane s = stage_lane(a, in_w, in_h);
+  constexpr int NV = 8;  // loads in flight per thread
+  for (int j0 = 0; j0 < s.nrow; j0 += NV) {
+    uint32_t v[NV];
+    sep2d_issue<DT, NV>(a, s, j0, v);
+    sep2d_commit<DT, NV>(a, s, s_in, in_wp, j0, v);
+  }
+}
 
 __device__ __forceinline__ void sep2d_store(const Sep2dTile &a, const float (&acc)[8]) {
   const int tid = threadIdx.x;
@@ -99,29 +129,15 @@ __device__ __forceinline__ void sep2d_store(const Sep2dTile &a, const float (&ac
   }
 }
 
-__device__ __forceinline__ void sep2d_tile(const Sep2dTile &a, const float *taps0, const float *taps1,
-                                           float *smem) {
-  float acc[8];
-  sep2d_compute(a, taps0, taps1, smem, acc);
-  sep2d_store(a, acc);
-}
-
-__device__ __forceinline__ void sep2d_compute(const Sep2dTile &a, const float *taps0, const float *taps1,
-                                              float *smem, float (&acc)[8]) {
+// The two factor passes over a staged tile (after the staging barrier); on
+// return thread t holds column t % 64, rows 8 (t / 64) .. +7 of the tile.
+__device__ __forceinline__ void sep2d_filter(const Sep2dTile &a, const float *taps0, const float *taps1,
+                                             float *smem, float (&acc)[8]) {
   const int in_w = S2_TX + a.k0 - 1;
   const int in_h = S2_TY + a.k1 - 1;
   const int in_wp = in_w | 1;
-  float *s_in = smem;                                     // in_h x in_wp
-  float *s_mid = smem + sep2d_mid_offset(a.k0, a.k1);    // in_h x 64
-  switch (a.dt_in) {
-  case IFM_F32: sep2d_stage<IFM_F32>(a, s_in, in_w, in_h, in_wp); break;
-  case IFM_BF16: sep2d_stage<IFM_BF16>(a, s_in, in_w, in_h, in_wp); break;
-  case IFM_F16: sep2d_stage<IFM_F16>(a, s_in, in_w, in_h, in_wp); break;
-  case IFM_F64: sep2d_stage<IFM_F64>(a, s_in, in_w, in_h, in_wp); break;
-  case IFM_N0F8: sep2d_stage<IFM_N0F8>(a, s_in, in_w, in_h, in_wp); break;
-  default: sep2d_stage<IFM_U8>(a, s_in, in_w, in_h, in_wp); break;
-  }
-  __syncthreads();
+  const float *s_in = smem;
+  float *s_mid = smem + sep2d_mid_offset(a.k0, a.k1);  // in_h x 64
   const int tid = threadIdx.x;
   // Axis-0 factor: item = (row, quad of 4 columns).
   for (int item = tid; item < in_h * 16; item += 256) {
@@ -140,8 +156,7 @@ __device__ __forceinline__ void sep2d_compute(const Sep2dTile &a, const float *t
       w[1] = w[2];
       w[2] = w[3];
     }
-    float4 r = make_float4(q[0], q[1], q[2], q[3]);
-    *(float4 *)(s_mid + row * S2_TX + cq) = r;
+    *(float4 *)(s_mid + row * S2_TX + cq) = make_float4(q[0], q[1], q[2], q[3]);
   }
   __syncthreads();
   // Axis-1 factor: each thread owns 8 consecutive rows of one column.
@@ -158,6 +173,31 @@ __device__ __forceinline__ void sep2d_compute(const Sep2dTile &a, const float *t
       if (j >= 0 && j < a.k1) acc[o] = fmaf(taps1[j], v, acc[o]);
     }
   }
+}
+
+// Stages and filters one tile (the caller owns the barrier before LDS is reused).
+__device__ __forceinline__ void sep2d_compute(const Sep2dTile &a, const float *taps0, const float *taps1,
+                                              float *smem, float (&acc)[8]) {
+  const int in_w = S2_TX + a.k0 - 1;
+  const int in_h = S2_TY + a.k1 - 1;
+  const int in_wp = in_w | 1;
+  switch (a.dt_in) {
+  case IFM_F32: sep2d_stage<IFM_F32>(a, smem, in_w, in_h, in_wp); break;
+  case IFM_BF16: sep2d_stage<IFM_BF16>(a, smem, in_w, in_h, in_wp); break;
+  case IFM_F16: sep2d_stage<IFM_F16>(a, smem, in_w, in_h, in_wp); break;
+  case IFM_F64: sep2d_stage<IFM_F64>(a, smem, in_w, in_h, in_wp); break;
+  case IFM_N0F8: sep2d_stage<IFM_N0F8>(a, smem, in_w, in_h, in_wp); break;
+  default: sep2d_stage<IFM_U8>(a, smem, in_w, in_h, in_wp); break;
+  }
+  __syncthreads();
+  sep2d_filter(a, taps0, taps1, smem, acc);
+}
+
+__device__ __forceinline__ void sep2d_tile(const Sep2dTile &a, const float *taps0, const float *taps1,
+                                           float *smem) {
+  float acc[8];
+  sep2d_compute(a, taps0, taps1, smem, acc);
+  sep2d_store(a, acc);
 }
 
 }  // namespace ifm

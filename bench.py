@@ -191,6 +191,8 @@ def run_serving(args):
     Poisson arrivals at 0.8 x the batched saturation rate (latency)."""
     from imagefiltering import serving
     n = args.serve_requests
+    if n <= 0:
+        return None
     budget = 64 << 30
     res = {"requests": n, "hbm_budget_bytes": budget,
            "mix": "sides log-uniform [64,4096]; F32 or BF16; gaussian sigma~U[0.5,8] | sobel gradient pair | "

@@ -404,7 +404,8 @@ int ifm_imfilter_fir(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const in
       int64_t zero[MAXD] = {0, 0, 0, 0};
       HIPCHK(launch_fir1d(vi.d, zero, vo.d, of, vi.d.n, p, style, (float)fill, st));
     } else if (axes.size() == 3 && axes[0] == 0 && axes[1] == 1 && axes[2] == 2 &&
-               sep3d_supported((int)taps[0].size(), (int)taps[1].size(), (int)taps[2].size(), vi.d.dtype)) {
+               sep3d_supported((int)taps[0].size(), (int)taps[1].size(), (int)taps[2].size(), vi.d.dtype,
+                               vi.d.n[0] * vi.d.n[1])) {
       // all three factors in one pass, no temporaries
       HIPCHK(launch_sep3d(vi.d, vo.d, of, pass_of(0), pass_of(1), pass_of(2), style, (float)fill, st));
     } else {

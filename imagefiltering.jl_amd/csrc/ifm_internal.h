@@ -179,9 +179,9 @@ hipError_t launch_fir1d(const Dense4 &in, const int64_t *in_first, Dense4 &out, 
                         const int64_t *img_n, const FirPass &p, int style, float fill, hipStream_t st);
 hipError_t launch_sep2d(const Dense4 &in, Dense4 &out, const int64_t *out_first, const FirPass &p0,
                         const FirPass &p1, int style, float fill, hipStream_t st);
-// Three factors on axes 0, 1, 2 in one pass (axis-1 factor <= 17 taps, axis-2
-// factor odd and <= 17 taps, F32 or F64 input).
-bool sep3d_supported(int k0, int k1, int k2, int dt_in);
+// Three factors on axes 0, 1, 2 in one pass (input tile <= 4096 elements,
+// axis-2 factor odd and <= 9 taps, F32 or F64 input, an (x, y) plane < 4 GiB).
+bool sep3d_supported(int k0, int k1, int k2, int dt_in, int64_t plane_elems);
 hipError_t launch_sep3d(const Dense4 &in, Dense4 &out, const int64_t *out_first, const FirPass &p0, const FirPass &p1,
                         const FirPass &p2, int style, float fill, hipStream_t st);
 

@@ -172,7 +172,7 @@ hipError_t launch_fir1d(const Dense4 &in, const int64_t *in_first, Dense4 &out, 
 
 // ---------------------------------------------------------------------------
 // sep2d: fused pair of factors on canonical axes 0 and 1, batched over axes
-// 2 and 3.  An output tile of 64 x 16 pixels loads its halo'd input tile into
+// 2 and 3.  An output tile of 64 x 32 pixels loads its halo'd input tile into
 // LDS once (border map on load), runs the axis-0 factor into a second LDS
 // tile and the axis-1 factor from there, so HBM sees one read and one write.
 // ---------------------------------------------------------------------------
@@ -228,14 +228,15 @@ __global__ void __launch_bounds__(256) k_sep2d(Sep2dArgs a) {
 // sep3d: the three-factor cascade on axes 0, 1 and 2 in one pass.  A
 // workgroup owns a 64 x 32 column of outputs in (x, y) and up to S3_NZ output
 // planes; it marches over the S3_NZ + K2 - 1 input planes they need, filters
-// each along axes 0 and 1 (sep2d_compute, border map on load) and adds it into
-// the K2 pending output planes kept in registers, writing a plane as soon as
-// its last input plane has arrived.  HBM sees the input once (plus halos) and
-// the output once; there is no temporary.  Accumulation along axis 2 runs in
-// tap order, like the cascade's third pass.
+// each along axes 0 and 1 (sep2d_filter) and adds it into the K2 pending
+// output planes kept in registers, writing a plane once its last input plane
+// has arrived.  HBM sees the input once (plus halos) and the output once;
+// there is no temporary.  Accumulation along axis 2 runs in tap order, like
+// the cascade's third pass.
 // ---------------------------------------------------------------------------
 constexpr int S3_NZ = 32;
-constexpr int S3_MAXK2 = 17;
+constexpr int S3_MAXK2 = 9;  // the plane loop is unrolled K2 times
+constexpr int S3_NE = 16;  // staged elements per thread and plane
 
 struct Sep3dArgs {
   const void *in;
@@ -255,7 +256,9 @@ struct Sep3dArgs {
   float taps2[S3_MAXK2];
 };
 
-constexpr int S3_NV = 24;  // staged rows per thread: in_h <= 48, i.e. k1 <= 17
+// LDS words: the sep2d tile layout plus one word that absorbs the writes of
+// staging slots with nothing to store.
+inline size_t sep3d_lds_floats(int k0, int k1) { return sep2d_lds_floats(k0, k1) + 1; }
 
 template <int K2, int DT>
 __global__ void __launch_bounds__(256) k_sep3d(Sep3dArgs a) {
@@ -281,65 +284,100 @@ __global__ void __launch_bounds__(256) k_sep3d(Sep3dArgs a) {
   t.oy0 = blockIdx.y * S2_TY;
   t.x0 = a.f0 + t.ox0;
   t.y0 = a.f1 + t.oy0;
-  // When input plane q arrives, acc[s] holds output plane q - (K2 - 1) + s:
-  // plane q adds taps2[j] to output q - j, i.e. slot K2 - 1 - j; afterwards
-  // slot 0 is complete and the window shifts by one (static register indices).
-  float acc[K2][8];
-#pragma unroll
-  for (int s = 0; s < K2; ++s)
-#pragma unroll
-    for (int r = 0; r < 8; ++r) acc[s][r] = 0.f;
-  // Software pipeline over input planes: the loads of plane q + 1 are issued
-  // (raw bits, into registers) before plane q is filtered, and committed to
-  // LDS at the top of the next step.
   const int in_w = S2_TX + a.k0 - 1, in_h = S2_TY + a.k1 - 1, in_wp = in_w | 1;
-  const StageLane ln = stage_lane(t, in_w, in_h);  // nrow <= S3_NV (k1 <= 17)
-  uint32_t v[S3_NV];
+  int *s_rows = (int *)(smem + sep2d_rows_offset(a.k0, a.k1));
+  const int dummy = sep2d_rows_offset(a.k0, a.k1) + in_h;
+  sep2d_rows(t, s_rows);
+  __syncthreads();
+  // Staging map, the same for every plane: slot i of the thread is element
+  // e = tid + 256 i of the in_h x in_w input tile, with its byte offset in the
+  // plane (a plane is < 4 GiB) and its LDS word.  Fill elements get the Fill
+  // value written once here and their slot writes go to the dummy word, as do
+  // slots past the tile, so a plane is staged by plain loads and LDS writes.
+  constexpr uint32_t ES = DT == IFM_F64 ? 8 : 4;
+  uint32_t off[S3_NE];
+  int lidx[S3_NE];
+  {
+    const int total = in_h * in_w;
+#pragma unroll
+    for (int i = 0; i < S3_NE; ++i) {
+      const int e = (int)threadIdx.x + 256 * i;
+      off[i] = 0;
+      lidx[i] = dummy;
+      if (e < total) {
+        const int r = e / in_w, c = e - r * in_w;
+        const int sy = s_rows[r];
+        const int sx = bmap(a.style, t.x0 + a.first0 + c, a.n0);
+        if (sy < 0 || sx < 0) {
+          smem[r * in_wp + c] = a.fill;
+        } else {
+          off[i] = ((uint32_t)sy * (uint32_t)a.n0 + (uint32_t)sx) * ES;
+          lidx[i] = r * in_wp + c;
+        }
+      }
+    }
+  }
+  using Raw = typename std::conditional<DT == IFM_F64, double, float>::type;
+  Raw v[S3_NE];
+  auto issue = [&](int zp) {
+    const char *pb = (const char *)a.in + ((int64_t)(p3 + a.f3) * a.n2 + zp) * a.in_plane * ES;
+#pragma unroll
+    for (int i = 0; i < S3_NE; ++i) v[i] = *(const Raw *)(pb + off[i]);
+  };
+  auto commit = [&]() {
+#pragma unroll
+    for (int i = 0; i < S3_NE; ++i) smem[lidx[i]] = (float)v[i];
+  };
+  auto store = [&](int od, const float (&acc)[8]) {
+    t.obase = ((int64_t)p3 * a.o2 + oz0 + od) * a.out_plane;
+    sep2d_store(t, acc);
+  };
+  // Output plane o accumulates input planes o .. o + K2 - 1 (taps 0 .. K2-1)
+  // in slot o mod K2.  The plane loop is unrolled K2 times so that slots are
+  // static registers: at input plane q (u = q mod K2) tap j goes to slot
+  // (u - j) mod K2, tap 0 starting the sum; output q - K2 + 1 is then
+  // complete in slot (u + 1) mod K2 and is written at the start of the next
+  // step (after that step's loads have been waited for), the last one at once.
+  float acc[K2][8];
   const int nq = nz + K2 - 1;
   auto plane_of = [&](int q) { return bmap(a.style, a.f2 + oz0 + a.first2 + q, a.n2); };
   int sz = plane_of(0);
-  if (sz >= 0) {
-    t.ibase = ((int64_t)(p3 + a.f3) * a.n2 + sz) * a.in_plane;
-    sep2d_issue<DT, S3_NV>(t, ln, 0, v);
-  }
-  for (int q = 0; q < nq; ++q) {
-    float sl[8];
-    if (sz >= 0) sep2d_commit<DT, S3_NV>(t, ln, smem, in_wp, 0, v);
-    __syncthreads();  // plane q staged
-    const int szn = q + 1 < nq ? plane_of(q + 1) : -1;
-    if (szn >= 0) {
-      t.ibase = ((int64_t)(p3 + a.f3) * a.n2 + szn) * a.in_plane;
-      sep2d_issue<DT, S3_NV>(t, ln, 0, v);
+  if (sz >= 0) issue(sz);
+  for (int q0 = 0; q0 < nq; q0 += K2) {
+#pragma unroll
+    for (int u = 0; u < K2; ++u) {
+      const int q = q0 + u;
+      if (q < nq) {
+        if (sz >= 0) commit();
+        __syncthreads();  // plane q staged
+        if (q >= K2) store(q - K2, acc[u]);
+        const int szn = q + 1 < nq ? plane_of(q + 1) : -1;
+        if (szn >= 0) issue(szn);
+        float sl[8];
+        if (sz < 0) {
+#pragma unroll
+          for (int r = 0; r < 8; ++r) sl[r] = a.fill01;
+        } else {
+          sep2d_filter(t, a.taps0, a.taps1, smem, sl);
+        }
+        __syncthreads();  // every wave is done with plane q's LDS tiles
+        sz = szn;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) acc[u][r] = a.taps2[0] * sl[r];
+#pragma unroll
+        for (int j = 1; j < K2; ++j)
+#pragma unroll
+          for (int r = 0; r < 8; ++r) acc[(u - j + K2) % K2][r] = fmaf(a.taps2[j], sl[r], acc[(u - j + K2) % K2][r]);
+        if (q == nq - 1) store(q - K2 + 1, acc[(u + 1) % K2]);
+      }
     }
-    if (sz < 0) {
-#pragma unroll
-      for (int r = 0; r < 8; ++r) sl[r] = a.fill01;
-    } else {
-      sep2d_filter(t, a.taps0, a.taps1, smem, sl);
-    }
-    __syncthreads();  // every wave is done with plane q's LDS tiles
-    sz = szn;
-#pragma unroll
-    for (int j = 0; j < K2; ++j)
-#pragma unroll
-      for (int r = 0; r < 8; ++r) acc[K2 - 1 - j][r] = fmaf(a.taps2[j], sl[r], acc[K2 - 1 - j][r]);
-    const int od = q - (K2 - 1);  // complete (slots of outputs below 0 are dropped)
-    if (od >= 0) {
-      t.obase = ((int64_t)p3 * a.o2 + oz0 + od) * a.out_plane;
-      sep2d_store(t, acc[0]);
-    }
-#pragma unroll
-    for (int s = 0; s + 1 < K2; ++s)
-#pragma unroll
-      for (int r = 0; r < 8; ++r) acc[s][r] = acc[s + 1][r];
-#pragma unroll
-    for (int r = 0; r < 8; ++r) acc[K2 - 1][r] = 0.f;
   }
 }
 
-bool sep3d_supported(int k0, int k1, int k2, int dt_in) {
-  return k0 <= S2_MAXK && S2_TY + k1 - 1 <= 2 * S3_NV && k2 <= S3_MAXK2 && (k2 & 1) &&
-         (dt_in == IFM_F32 || dt_in == IFM_F64);
+bool sep3d_supported(int k0, int k1, int k2, int dt_in, int64_t plane_elems) {
+  return k0 <= S2_MAXK && k1 <= S2_MAXK && (S2_TY + k1 - 1) * (S2_TX + k0 - 1) <= 256 * S3_NE &&
+         k2 <= S3_MAXK2 && (k2 & 1) && (dt_in == IFM_F32 || dt_in == IFM_F64) &&
+         plane_elems * (dt_in == IFM_F64 ? 8 : 4) < (int64_t(1) << 32);
 }
 
 template <int K2>
@@ -386,7 +424,7 @@ hipError_t launch_sep3d(const Dense4 &in, Dense4 &out, const int64_t *out_first,
   for (int t = 0; t < p0.len; ++t) a.taps0[t] = p0.taps[t];
   for (int t = 0; t < p1.len; ++t) a.taps1[t] = p1.taps[t];
   for (int t = 0; t < p2.len; ++t) a.taps2[t] = p2.taps[t];
-  const size_t lds = sizeof(float) * sep2d_lds_floats(a.k0, a.k1);
+  const size_t lds = sizeof(float) * sep3d_lds_floats(a.k0, a.k1);
   const int64_t gz = (int64_t)a.nzc * a.n3;
   if (gz > 65535) return hipErrorInvalidValue;
   dim3 grid((a.o0 + S2_TX - 1) / S2_TX, (a.o1 + S2_TY - 1) / S2_TY, (unsigned)gz);
@@ -397,10 +435,6 @@ hipError_t launch_sep3d(const Dense4 &in, Dense4 &out, const int64_t *out_first,
   case 5: sep3d_go<5>(a, f64, grid, lds, st); break;
   case 7: sep3d_go<7>(a, f64, grid, lds, st); break;
   case 9: sep3d_go<9>(a, f64, grid, lds, st); break;
-  case 11: sep3d_go<11>(a, f64, grid, lds, st); break;
-  case 13: sep3d_go<13>(a, f64, grid, lds, st); break;
-  case 15: sep3d_go<15>(a, f64, grid, lds, st); break;
-  case 17: sep3d_go<17>(a, f64, grid, lds, st); break;
   default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

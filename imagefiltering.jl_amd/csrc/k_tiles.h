@@ -35,13 +35,17 @@ struct Sep2dTile {
   float fill;
 };
 
-// LDS: input tile (odd pitch), then the mid tile at a 16-byte boundary.
+// LDS: input tile (odd pitch), the mid tile at a 16-byte boundary, then the
+// row table (border-mapped input row of each tile row, -1 for Fill).
 __host__ __device__ inline int sep2d_mid_offset(int k0, int k1) {
   const int in_h = S2_TY + k1 - 1, in_w = S2_TX + k0 - 1;
   return (in_h * (in_w | 1) + 3) & ~3;
 }
+__host__ __device__ inline int sep2d_rows_offset(int k0, int k1) {
+  return sep2d_mid_offset(k0, k1) + (S2_TY + k1 - 1) * S2_TX;
+}
 __host__ __device__ inline size_t sep2d_lds_floats(int k0, int k1) {
-  return (size_t)sep2d_mid_offset(k0, k1) + (size_t)(S2_TY + k1 - 1) * S2_TX;
+  return (size_t)sep2d_rows_offset(k0, k1) + (size_t)(S2_TY + k1 - 1);
 }
 
 // Raw element bits (F64 is narrowed at once) and their value: a load is not
@@ -77,42 +81,57 @@ __device__ __forceinline__ StageLane stage_lane(const Sep2dTile &a, int in_w, in
   return s;
 }
 
-// Loads of rows j0 .. j0+NV-1 of the thread (Fill positions load element 0).
+// Fills the row table of the tile (the caller owns the barrier after it).
+__device__ __forceinline__ void sep2d_rows(const Sep2dTile &a, int *s_rows) {
+  const int in_h = S2_TY + a.k1 - 1;
+  for (int i = threadIdx.x; i < in_h; i += blockDim.x) s_rows[i] = bmap(a.style, a.y0 + a.first1 + i, a.n1);
+}
+
+// Row of the table as a wave-uniform value (the thread's rows are uniform).
+__device__ __forceinline__ int sep2d_row(const int *s_rows, int r) {
+  return __builtin_amdgcn_readfirstlane(s_rows[r]);
+}
+
+// Loads of rows j0 .. j0+NV-1 of the thread.  Fill rows load nothing and set
+// their bit in `fillrows` (wave-uniform); Fill columns load element 0 of the row.
 template <int DT, int NV>
-__device__ __forceinline__ void sep2d_issue(const Sep2dTile &a, const StageLane &s, int j0, uint32_t (&v)[NV]) {
+__device__ __forceinline__ void sep2d_issue(const Sep2dTile &a, const StageLane &s, const int *s_rows, int j0,
+                                            uint32_t (&v)[NV], uint32_t &fillrows) {
   const int sxc = s.sx < 0 ? 0 : s.sx;
+  fillrows = 0;
 #pragma unroll
   for (int j = 0; j < NV; ++j) {
     const int jj = j0 + j;
     if (jj < s.nrow) {  // wave-uniform
-      const int sy = bmap(a.style, a.y0 + a.first1 + s.rh + 2 * jj, a.n1);
-      v[j] = raw_ld<DT>(a.in, a.ibase + (sy < 0 ? 0 : (int64_t)sy * a.n0 + sxc));
+      const int sy = sep2d_row(s_rows, s.rh + 2 * jj);
+      if (sy >= 0) v[j] = raw_ld<DT>(a.in, a.ibase + (int64_t)sy * a.n0 + sxc);
+      else fillrows |= 1u << j;
     }
   }
 }
 
 template <int DT, int NV>
 __device__ __forceinline__ void sep2d_commit(const Sep2dTile &a, const StageLane &s, float *s_in, int in_wp, int j0,
-                                             const uint32_t (&v)[NV]) {
+                                             const uint32_t (&v)[NV], uint32_t fillrows) {
   if (!s.colok) return;
+  const bool fillcol = s.sx < 0;
 #pragma unroll
   for (int j = 0; j < NV; ++j) {
     const int jj = j0 + j;
-    if (jj < s.nrow) {
-      const int sy = bmap(a.style, a.y0 + a.first1 + s.rh + 2 * jj, a.n1);
-      s_in[(s.rh + 2 * jj) * in_wp + s.c] = (sy < 0 || s.sx < 0) ? a.fill : raw_val<DT>(v[j]);
-    }
+    if (jj < s.nrow) s_in[(s.rh + 2 * jj) * in_wp + s.c] = (((fillrows >> j) & 1u) || fillcol) ? a.fill : raw_val<DT>(v[j]);
   }
 }
 
+// Stages the tile (row table already filled and visible).
 template <int DT>
-__device__ __forceinline__ void sep2d_stage(const Sep2dTile &a, float *s_in, int in_w, int in_h, int in_wp) {
+__device__ __forceinline__ void sep2d_stage(const Sep2dTile &a, const int *s_rows, float *s_in, int in_w, int in_h,
+                                            int in_wp) {
   const StageLane s = stage_lane(a, in_w, in_h);
   constexpr int NV = 8;  // loads in flight per thread
   for (int j0 = 0; j0 < s.nrow; j0 += NV) {
-    uint32_t v[NV];
-    sep2d_issue<DT, NV>(a, s, j0, v);
-    sep2d_commit<DT, NV>(a, s, s_in, in_wp, j0, v);
+    uint32_t v[NV], fillrows;
+    sep2d_issue<DT, NV>(a, s, s_rows, j0, v, fillrows);
+    sep2d_commit<DT, NV>(a, s, s_in, in_wp, j0, v, fillrows);
   }
 }
 
@@ -159,19 +178,23 @@ __device__ __forceinline__ void sep2d_filter(const Sep2dTile &a, const float *ta
     *(float4 *)(s_mid + row * S2_TX + cq) = make_float4(q[0], q[1], q[2], q[3]);
   }
   __syncthreads();
-  // Axis-1 factor: each thread owns 8 consecutive rows of one column.
+  // Axis-1 factor: each thread owns 8 consecutive rows of one column and
+  // slides an 8-value window down the column, one LDS read per tap.
   const int col = tid & 63, r0 = (tid >> 6) * 8;
 #pragma unroll
   for (int o = 0; o < 8; ++o) acc[o] = 0.f;
   const float *src = s_mid + r0 * S2_TX + col;
-#pragma unroll 4
-  for (int u = 0; u < a.k1 + 7; ++u) {
-    const float v = src[u * S2_TX];
+  float w[8];
 #pragma unroll
-    for (int o = 0; o < 8; ++o) {
-      const int j = u - o;
-      if (j >= 0 && j < a.k1) acc[o] = fmaf(taps1[j], v, acc[o]);
-    }
+  for (int o = 0; o < 7; ++o) w[o] = src[o * S2_TX];
+#pragma unroll 8
+  for (int t = 0; t < a.k1; ++t) {
+    w[7] = src[(t + 7) * S2_TX];
+    const float k = taps1[t];
+#pragma unroll
+    for (int o = 0; o < 8; ++o) acc[o] = fmaf(k, w[o], acc[o]);
+#pragma unroll
+    for (int o = 0; o < 7; ++o) w[o] = w[o + 1];
   }
 }
 
@@ -181,13 +204,16 @@ __device__ __forceinline__ void sep2d_compute(const Sep2dTile &a, const float *t
   const int in_w = S2_TX + a.k0 - 1;
   const int in_h = S2_TY + a.k1 - 1;
   const int in_wp = in_w | 1;
+  int *s_rows = (int *)(smem + sep2d_rows_offset(a.k0, a.k1));
+  sep2d_rows(a, s_rows);
+  __syncthreads();
   switch (a.dt_in) {
-  case IFM_F32: sep2d_stage<IFM_F32>(a, smem, in_w, in_h, in_wp); break;
-  case IFM_BF16: sep2d_stage<IFM_BF16>(a, smem, in_w, in_h, in_wp); break;
-  case IFM_F16: sep2d_stage<IFM_F16>(a, smem, in_w, in_h, in_wp); break;
-  case IFM_F64: sep2d_stage<IFM_F64>(a, smem, in_w, in_h, in_wp); break;
-  case IFM_N0F8: sep2d_stage<IFM_N0F8>(a, smem, in_w, in_h, in_wp); break;
-  default: sep2d_stage<IFM_U8>(a, smem, in_w, in_h, in_wp); break;
+  case IFM_F32: sep2d_stage<IFM_F32>(a, s_rows, smem, in_w, in_h, in_wp); break;
+  case IFM_BF16: sep2d_stage<IFM_BF16>(a, s_rows, smem, in_w, in_h, in_wp); break;
+  case IFM_F16: sep2d_stage<IFM_F16>(a, s_rows, smem, in_w, in_h, in_wp); break;
+  case IFM_F64: sep2d_stage<IFM_F64>(a, s_rows, smem, in_w, in_h, in_wp); break;
+  case IFM_N0F8: sep2d_stage<IFM_N0F8>(a, s_rows, smem, in_w, in_h, in_wp); break;
+  default: sep2d_stage<IFM_U8>(a, s_rows, smem, in_w, in_h, in_wp); break;
   }
   __syncthreads();
   sep2d_filter(a, taps0, taps1, smem, acc);

@@ -11,8 +11,8 @@ namespace ifm {
 // 256 threads computes a 64 x 32 output tile:
 //   1. the (32 + k1 - 1) x (64 + k0 - 1) input tile is loaded once into LDS,
 //      border map on load; each thread owns one column and every other row.
-//      Loads keep the raw element bits and are issued in groups before any
-//      is used (issue / commit), so several are in flight per thread;
+//      Loads keep the raw element bits and are issued in groups of 8 before
+//      any is used (issue / commit), so several are in flight per thread;
 //   2. the axis-0 factor runs from registers: a thread slides over k0 + 3
 //      LDS values to produce 4 consecutive outputs of one row (mid tile);
 //   3. the axis-1 factor likewise produces 8 consecutive rows of one column.
@@ -87,26 +87,25 @@ __device__ __forceinline__ void sep2d_rows(const Sep2dTile &a, int *s_rows) {
   for (int i = threadIdx.x; i < in_h; i += blockDim.x) s_rows[i] = bmap(a.style, a.y0 + a.first1 + i, a.n1);
 }
 
-// Row of the table as a wave-uniform value (the thread's rows are uniform).
-__device__ __forceinline__ int sep2d_row(const int *s_rows, int r) {
-  return __builtin_amdgcn_readfirstlane(s_rows[r]);
-}
-
-// Loads of rows j0 .. j0+NV-1 of the thread.  Fill rows load nothing and set
-// their bit in `fillrows` (wave-uniform); Fill columns load element 0 of the row.
+// Loads of rows j0 .. j0+NV-1 of the thread.  The row table is read for all
+// of them first (one LDS round trip), then every load is issued; Fill rows
+// and columns load an element of row 0 / column 0 and set their bit in
+// `fillrows` (wave-uniform), which the commit replaces by the Fill value.
 template <int DT, int NV>
 __device__ __forceinline__ void sep2d_issue(const Sep2dTile &a, const StageLane &s, const int *s_rows, int j0,
                                             uint32_t (&v)[NV], uint32_t &fillrows) {
   const int sxc = s.sx < 0 ? 0 : s.sx;
+  const int last = 2 * s.nrow - 2 + s.rh;  // the thread's last row
+  int sy[NV];
+#pragma unroll
+  for (int j = 0; j < NV; ++j) sy[j] = s_rows[min(s.rh + 2 * (j0 + j), last)];
   fillrows = 0;
 #pragma unroll
   for (int j = 0; j < NV; ++j) {
-    const int jj = j0 + j;
-    if (jj < s.nrow) {  // wave-uniform
-      const int sy = sep2d_row(s_rows, s.rh + 2 * jj);
-      if (sy >= 0) v[j] = raw_ld<DT>(a.in, a.ibase + (int64_t)sy * a.n0 + sxc);
-      else fillrows |= 1u << j;
-    }
+    const int y = __builtin_amdgcn_readfirstlane(sy[j]);
+    fillrows |= (y < 0 ? 1u : 0u) << j;
+    if (j0 + j < s.nrow)  // wave-uniform
+      v[j] = raw_ld<DT>(a.in, a.ibase + (int64_t)(y < 0 ? 0 : y) * a.n0 + sxc);
   }
 }
 

@@ -25,30 +25,41 @@ struct PadArgs {
   int64_t total;
 };
 
+// Rows of the padded output (axis 0) are the unit of work: a workgroup maps
+// its row's axes 1..3 through the border once (wave-uniform), and each thread
+// then maps and copies PAD_EPT elements of the row, 256 apart, so reads and
+// writes are coalesced and no per-element division is needed.
+constexpr int PAD_EPT = 4;
+
+template <int ES>  // raw copy of ES-byte elements (same eltype in and out); 0: converting copy
 __global__ void __launch_bounds__(256) k_pad(PadArgs a) {
-  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < a.total;
-       q += (int64_t)gridDim.x * blockDim.x) {
-    int64_t r = q;
-    int i0 = (int)(r % a.p[0]); r /= a.p[0];
-    int i1 = (int)(r % a.p[1]); r /= a.p[1];
-    int i2 = (int)(r % a.p[2]); r /= a.p[2];
-    int i3 = (int)r;
-    int s0 = bmap(a.style, i0 - a.lo[0], a.n[0]);
-    int s1 = bmap(a.style, i1 - a.lo[1], a.n[1]);
-    int s2 = bmap(a.style, i2 - a.lo[2], a.n[2]);
-    int s3 = bmap(a.style, i3 - a.lo[3], a.n[3]);
-    bool isfill = (s0 | s1 | s2 | s3) < 0;
-    int64_t src = s0 + (int64_t)a.n[0] * (s1 + (int64_t)a.n[1] * (s2 + (int64_t)a.n[2] * s3));
-    if (a.dt_in == a.dt_out) {
-      switch (a.esize) {
-      case 1: ((uint8_t *)a.out)[q] = isfill ? (uint8_t)a.fill_bits : ((const uint8_t *)a.in)[src]; break;
-      case 2: ((uint16_t *)a.out)[q] = isfill ? (uint16_t)a.fill_bits : ((const uint16_t *)a.in)[src]; break;
-      case 4: ((uint32_t *)a.out)[q] = isfill ? (uint32_t)a.fill_bits : ((const uint32_t *)a.in)[src]; break;
-      default: ((uint64_t *)a.out)[q] = isfill ? a.fill_bits : ((const uint64_t *)a.in)[src]; break;
+  using U = typename std::conditional<ES == 1, uint8_t,
+            typename std::conditional<ES == 2, uint16_t,
+            typename std::conditional<ES == 4, uint32_t, uint64_t>::type>::type>::type;
+  const int rows = a.p[1] * a.p[2] * a.p[3];
+  for (int row = blockIdx.y; row < rows; row += gridDim.y) {
+    const int i1 = row % a.p[1], t = row / a.p[1];
+    const int i2 = t % a.p[2], i3 = t / a.p[2];
+    const int s1 = bmap(a.style, i1 - a.lo[1], a.n[1]);
+    const int s2 = bmap(a.style, i2 - a.lo[2], a.n[2]);
+    const int s3 = bmap(a.style, i3 - a.lo[3], a.n[3]);
+    const bool rowfill = (s1 | s2 | s3) < 0;
+    const int64_t srow = rowfill ? 0 : (s1 + (int64_t)a.n[1] * (s2 + (int64_t)a.n[2] * s3)) * a.n[0];
+    const int64_t orow = (int64_t)row * a.p[0];
+    const int x0 = blockIdx.x * (256 * PAD_EPT) + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < PAD_EPT; ++k) {
+      const int x = x0 + 256 * k;
+      if (x < a.p[0]) {
+        const int s0 = bmap(a.style, x - a.lo[0], a.n[0]);
+        const bool isfill = rowfill || s0 < 0;
+        if constexpr (ES > 0) {
+          ((U *)a.out)[orow + x] = isfill ? (U)a.fill_bits : ((const U *)a.in)[srow + s0];
+        } else {
+          const double v = isfill ? a.fill : load_d(a.in, srow + s0, a.dt_in);
+          store_d(a.out, orow + x, a.dt_out, v);
+        }
       }
-    } else {
-      double v = isfill ? a.fill : load_d(a.in, src, a.dt_in);
-      store_d(a.out, q, a.dt_out, v);
     }
   }
 }
@@ -83,9 +94,16 @@ hipError_t launch_pad(const Dense4 &in, Dense4 &out, const int64_t *lo, int styl
   a.fill_bits = fill_bits_for(out.dtype, fill);
   a.total = out.count();
   if (a.total == 0) return hipSuccess;
-  int64_t blocks = (a.total + 255) / 256;
-  if (blocks > 65536) blocks = 65536;
-  hipLaunchKernelGGL(k_pad, dim3((unsigned)blocks), dim3(256), 0, st, a);
+  const int64_t rows = out.n[1] * out.n[2] * out.n[3];
+  if (rows > INT32_MAX || out.n[0] > INT32_MAX) return hipErrorInvalidValue;
+  dim3 grid((unsigned)((out.n[0] + 256 * PAD_EPT - 1) / (256 * PAD_EPT)), (unsigned)std::min<int64_t>(rows, 65535));
+  if (in.dtype != out.dtype) hipLaunchKernelGGL(k_pad<0>, grid, dim3(256), 0, st, a);
+  else switch (a.esize) {
+  case 1: hipLaunchKernelGGL(k_pad<1>, grid, dim3(256), 0, st, a); break;
+  case 2: hipLaunchKernelGGL(k_pad<2>, grid, dim3(256), 0, st, a); break;
+  case 4: hipLaunchKernelGGL(k_pad<4>, grid, dim3(256), 0, st, a); break;
+  default: hipLaunchKernelGGL(k_pad<8>, grid, dim3(256), 0, st, a); break;
+  }
   return hipGetLastError();
 }
 
@@ -110,36 +128,66 @@ struct Fir1dArgs {
   float taps[MAXTAPS];
 };
 
+// Rows of the output along axis 0 are the unit of work (as in k_pad): the
+// row's coordinates on axes 1..3 are formed once per workgroup, and a thread
+// computes FIR_EPT outputs of the row, 256 apart.  Along axis 0 the taps read
+// consecutive elements; along another axis they read one element per row of
+// that axis, and the wave's lanes read consecutive addresses.
+constexpr int FIR_EPT = 4;
+
+template <int DT>  // input eltype; -1: read at run time
+__device__ __forceinline__ float fir_ld(const Fir1dArgs &a, int64_t i) {
+  if constexpr (DT == IFM_F32) return ((const float *)a.in)[i];
+  else if constexpr (DT == IFM_F64) return (float)((const double *)a.in)[i];
+  else return load_f(a.in, i, a.dt_in);
+}
+
+template <int DT>
 __global__ void __launch_bounds__(256) k_fir1d(Fir1dArgs a) {
-  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < a.total;
-       q += (int64_t)gridDim.x * blockDim.x) {
-    int64_t r = q;
+  const int rows = a.out_n[1] * a.out_n[2] * a.out_n[3];
+  const int64_t st1 = a.in_n[0], st2 = st1 * a.in_n[1], st3 = st2 * a.in_n[2];
+  const int64_t stride[MAXD] = {1, st1, st2, st3};
+  const int n = a.in_n[a.axis];
+  const int64_t astride = stride[a.axis];
+  for (int row = blockIdx.y; row < rows; row += gridDim.y) {
     int o[MAXD];
-    o[0] = (int)(r % a.out_n[0]); r /= a.out_n[0];
-    o[1] = (int)(r % a.out_n[1]); r /= a.out_n[1];
-    o[2] = (int)(r % a.out_n[2]); r /= a.out_n[2];
-    o[3] = (int)r;
-    int64_t base = 0, stride = 1, astride = 1;
+    o[1] = row % a.out_n[1];
+    const int t = row / a.out_n[1];
+    o[2] = t % a.out_n[2];
+    o[3] = t / a.out_n[2];
+    int64_t base_u = 0;  // input offset of the row on the non-filtered axes 1..3
 #pragma unroll
-    for (int d = 0; d < MAXD; ++d) {
-      if (d == a.axis) astride = stride;
-      else base += (int64_t)(o[d] + a.shift[d]) * stride;
-      stride *= a.in_n[d];
-    }
-    int x = o[a.axis] + a.first;
-    int n = a.in_n[a.axis];
-    float acc = 0.f;
-    if (x >= 0 && x + a.len <= n) {
-      const int64_t b0 = base + (int64_t)x * astride;
-      for (int t = 0; t < a.len; ++t) acc = fmaf(a.taps[t], load_f(a.in, b0 + t * astride, a.dt_in), acc);
-    } else {
-      for (int t = 0; t < a.len; ++t) {
-        int s = bmap(a.style, x + t, n);
-        float v = s < 0 ? a.fill : load_f(a.in, base + (int64_t)s * astride, a.dt_in);
-        acc = fmaf(a.taps[t], v, acc);
+    for (int d = 1; d < MAXD; ++d)
+      if (d != a.axis) base_u += (int64_t)(o[d] + a.shift[d]) * stride[d];
+    const int xu = a.axis == 0 ? 0 : o[a.axis] + a.first;
+    const int64_t orow = (int64_t)row * a.out_n[0];
+    const int x0 = blockIdx.x * (256 * FIR_EPT) + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < FIR_EPT; ++k) {
+      const int o0 = x0 + 256 * k;
+      if (o0 >= a.out_n[0]) break;
+      int64_t base;
+      int x;
+      if (a.axis == 0) {
+        base = base_u;
+        x = o0 + a.first;
+      } else {
+        base = base_u + o0 + a.shift[0];
+        x = xu;
       }
+      float acc = 0.f;
+      if (x >= 0 && x + a.len <= n) {
+        const int64_t b0 = base + (int64_t)x * astride;
+        for (int tt = 0; tt < a.len; ++tt) acc = fmaf(a.taps[tt], fir_ld<DT>(a, b0 + tt * astride), acc);
+      } else {
+        for (int tt = 0; tt < a.len; ++tt) {
+          const int s = bmap(a.style, x + tt, n);
+          const float v = s < 0 ? a.fill : fir_ld<DT>(a, base + (int64_t)s * astride);
+          acc = fmaf(a.taps[tt], v, acc);
+        }
+      }
+      store_f(a.out, orow + o0, a.dt_out, acc);
     }
-    store_f(a.out, q, a.dt_out, acc);
   }
 }
 
@@ -164,9 +212,12 @@ hipError_t launch_fir1d(const Dense4 &in, const int64_t *in_first, Dense4 &out, 
   a.total = out.count();
   for (int t = 0; t < p.len; ++t) a.taps[t] = p.taps[t];
   if (a.total == 0) return hipSuccess;
-  int64_t blocks = (a.total + 255) / 256;
-  if (blocks > 1 << 20) blocks = 1 << 20;
-  hipLaunchKernelGGL(k_fir1d, dim3((unsigned)blocks), dim3(256), 0, st, a);
+  const int64_t rows = out.n[1] * out.n[2] * out.n[3];
+  if (rows > INT32_MAX) return hipErrorInvalidValue;
+  dim3 grid((unsigned)((out.n[0] + 256 * FIR_EPT - 1) / (256 * FIR_EPT)), (unsigned)std::min<int64_t>(rows, 65535));
+  if (in.dtype == IFM_F32) hipLaunchKernelGGL(k_fir1d<IFM_F32>, grid, dim3(256), 0, st, a);
+  else if (in.dtype == IFM_F64) hipLaunchKernelGGL(k_fir1d<IFM_F64>, grid, dim3(256), 0, st, a);
+  else hipLaunchKernelGGL(k_fir1d<-1>, grid, dim3(256), 0, st, a);
   return hipGetLastError();
 }
 

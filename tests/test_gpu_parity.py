@@ -80,6 +80,22 @@ def test_padarray_random_all_styles_bit_exact(dtype):
             np.testing.assert_array_equal(np.asarray(P.data).astype(np.float64), want, err_msg="%s %s" % (style, shape))
 
 
+def test_padarray_long_rows_and_many_rows():
+    # rows longer than one workgroup's span, and more rows than one grid column
+    rng = np.random.default_rng(12)
+    cases = [((5000,), (7,), (1030,)), ((1100, 3), (3, 2), (1, 4)), ((2, 300, 250), (1, 2, 3), (2, 1, 0))]
+    for shape, lo, hi in cases:
+        x = np.asfortranarray(rng.random(shape).astype(np.float32))
+        for style in STYLES + ["fill"]:
+            b = IF.Fill(0.25, lo, hi) if style == "fill" else IF.Pad(style, lo, hi)
+            P = IF.padarray(x, b)
+            want = O.padarray(x.astype(np.float64), obord(b) if style == "fill" else style, list(lo), list(hi))
+            np.testing.assert_array_equal(np.asarray(P.data).astype(np.float64), want, err_msg="%s %s" % (style, shape))
+        P = IF.padarray(IF.Float64, x, IF.Pad("circular", lo, hi))
+        want = O.padarray(x.astype(np.float64), "circular", list(lo), list(hi))
+        np.testing.assert_array_equal(P.data, want)
+
+
 def test_padarray_conversion_and_layout():
     x = np.asfortranarray(np.random.default_rng(1).random((7, 5)).astype(np.float32))
     P = IF.padarray(IF.Float64, x, IF.Pad("symmetric", (3, 1), (2, 4)))
@@ -162,6 +178,37 @@ def test_separable_3d_fused_many_tiles():
             out = IF.imfilter(vol, kern, b)
             want = O.imfilter_factors(vol.astype(np.float64), factors_of(kern), obord(b))
             assert relerr(out, want) <= TOL, (dt, b)
+
+
+def test_single_axis_factors_long_rows():
+    # one factor per call along axis 0, 1 or 2 (the per-axis pass), rows
+    # longer than a workgroup's span, a 3-D cascade whose axis-2 factor is too
+    # long for the fused pass, and the identity (a plain copy)
+    rng = np.random.default_rng(42)
+    sig = rng.random(5000).astype(np.float32)
+    k7 = IF.centered(np.array([0.05, 0.1, 0.2, 0.3, 0.2, 0.1, 0.05]))
+    for b in STYLES + [IF.Fill(0.5)]:
+        out = IF.imfilter(sig, KernelFactors.kernelfactors((k7,)), b)
+        want = O.imfilter_factors(sig.astype(np.float64), [(0, list(k7.data), -3)], obord(b))
+        assert relerr(out, want) <= TOL, b
+    vol = np.asfortranarray(rng.random((1500, 7, 9)).astype(np.float32))
+    for ax in range(3):
+        f = [np.array([1.0])] * 3
+        f[ax] = np.array([0.25, 0.5, 0.25])
+        kern = KernelFactors.kernelfactors(tuple(IF.centered(x) for x in f))
+        for b in ["reflect", "circular"]:
+            out = IF.imfilter(vol, kern, b)
+            want = O.imfilter_factors(vol.astype(np.float64), factors_of(kern), obord(b))
+            assert relerr(out, want) <= TOL, (ax, b)
+    v3 = np.asfortranarray(rng.random((40, 30, 50)).astype(np.float32))
+    kern = KernelFactors.gaussian((1.0, 1.5, 3.0))  # 5, 9 and 13 taps
+    for b in ["symmetric", IF.Fill(0.0)]:
+        out = IF.imfilter(v3, kern, b)
+        want = O.imfilter_factors(v3.astype(np.float64), factors_of(kern), obord(b))
+        assert relerr(out, want) <= TOL, b
+    ident = KernelFactors.kernelfactors((IF.centered(np.array([1.0])), IF.centered(np.array([1.0]))))
+    img = np.asfortranarray(rng.random((1300, 20)).astype(np.float32))
+    np.testing.assert_array_equal(IF.imfilter(img, ident), img.astype(np.float64))
 
 
 def test_row_major_input_same_as_column_major():

@@ -212,7 +212,7 @@ struct MfmaArgs {
   int dt_in, dt_out;
   float fill;
 };
-bool dense_mfma_supported(const DenseKernel &k, const Dense4 &in);
+bool dense_mfma_supported(const DenseKernel &k, const Dense4 &in, const Dense4 &out);
 void dense_mfma_table(const DenseKernel &k, std::vector<uint16_t> &table, float *kinv);
 // Arguments of one image (ntiles = its tiles); false if the shape is unsupported.
 bool dense_mfma_args(const Dense4 &in, Dense4 &out, const int64_t *out_first, const DenseKernel &k,

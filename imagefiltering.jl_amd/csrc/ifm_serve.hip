@@ -368,7 +368,7 @@ int build_batch(ifm_ctx *ctx, const ifm_request *reqs, const int *idx, int n, co
       const ifm_request &q = reqs[idx[i]];
       if (q.op != IFM_OP_DENSE) continue;
       dks[i] = dense_of(q.kernel);
-      if (!dense_mfma_supported(dks[i], vin[i]))
+      if (!dense_mfma_supported(dks[i], vin[i], vout[i]))
         return sfail(ctx, IFM_E_UNSUPPORTED, "dense requests need a 2-D kernel of at most 49 x 32 taps");
       byvar[dense_mfma_variant(dks[i])].push_back(i);
     }

@@ -363,31 +363,106 @@ __device__ __forceinline__ void mf_chunk(const uint32_t *lbase, const f16x8 (&th
   }
 }
 
+// Both chunks of a 16-row kernel (one K block) as one sweep: kernel row U of
+// block b reads fragment row rho = b + U (chunk 1's row r is fragment row
+// r + 8), so fragment rows 0 .. 22 are read once each and every one between 7
+// and 15 feeds all 8 blocks; the pipeline fills and drains once instead of
+// once per chunk.
+template <class G>
+__device__ __forceinline__ void mf_rows16(const uint32_t *lbase, const f16x8 (&ahi)[MF_UCH],
+                                          const f16x8 (&alo)[MF_UCH], const f16x8 (&bhi)[MF_UCH],
+                                          const f16x8 (&blo)[MF_UCH], f32x4 (&acc)[MF_B]) {
+  constexpr int NR = MF_B + 2 * MF_UCH - 1;
+  const u32x4 *src0 = (const u32x4 *)lbase;
+  u32x4 w0 = src0[0], w1 = src0[1];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    u32x4 n0 = w0, n1 = w1;
+    if (r + 1 < NR) {
+      const u32x4 *src = (const u32x4 *)(lbase + mf_slot(r + 1, G::RB) * G::P);
+      n0 = src[0];
+      n1 = src[1];
+    }
+    u32x4 ph, pl;
+    ph[0] = __builtin_amdgcn_perm(w0[1], w0[0], 0x05040100u);
+    ph[1] = __builtin_amdgcn_perm(w0[3], w0[2], 0x05040100u);
+    ph[2] = __builtin_amdgcn_perm(w1[1], w1[0], 0x05040100u);
+    ph[3] = __builtin_amdgcn_perm(w1[3], w1[2], 0x05040100u);
+    pl[0] = __builtin_amdgcn_perm(w0[1], w0[0], 0x07060302u);
+    pl[1] = __builtin_amdgcn_perm(w0[3], w0[2], 0x07060302u);
+    pl[2] = __builtin_amdgcn_perm(w1[1], w1[0], 0x07060302u);
+    pl[3] = __builtin_amdgcn_perm(w1[3], w1[2], 0x07060302u);
+    const f16x8 fhi = __builtin_bit_cast(f16x8, ph);
+    const f16x8 flo = __builtin_bit_cast(f16x8, pl);
+#pragma unroll
+    for (int b = 0; b < MF_B; ++b) {
+      const int u = r - b;
+      if (u >= 0 && u < 2 * MF_UCH)
+        acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fhi, u < MF_UCH ? ahi[u] : bhi[u - MF_UCH], acc[b], 0, 0, 0);
+    }
+#pragma unroll
+    for (int b = 0; b < MF_B; ++b) {
+      const int u = r - b;
+      if (u >= 0 && u < 2 * MF_UCH)
+        acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(flo, u < MF_UCH ? ahi[u] : bhi[u - MF_UCH], acc[b], 0, 0, 0);
+    }
+#pragma unroll
+    for (int b = 0; b < MF_B; ++b) {
+      const int u = r - b;
+      if (u >= 0 && u < 2 * MF_UCH)
+        acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fhi, u < MF_UCH ? alo[u] : blo[u - MF_UCH], acc[b], 0, 0, 0);
+    }
+    w0 = n0;
+    w1 = n1;
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
 // D layout: lane holds rows yy = 4*(lane/16) + i of column a = lane%16, i.e.
-// output rows oy0 + b + 8*yy.
+// output rows oy0 + b + 8*yy.  Stores go through a buffer descriptor of the
+// output plane: an output outside the image (or every output, keep = false)
+// gets an offset past the descriptor's range, which the hardware drops.  So
+// every path issues exactly 32 stores per lane, and the wait for the next
+// tile's loads (issued before these stores) need not wait for the stores.
+constexpr uint32_t MF_DROP = 0x80000000u;  // beyond any plane (< 2^31 bytes)
+
 template <int DT>
 __device__ __forceinline__ void mf_store_t(const MfmaArgs &a, const MfTile &T, const f32x4 (&acc)[MF_B], float osc,
-                                           int ox, int lgrp) {
-  const bool full = T.oy0 + MF_TY <= a.o1;
+                                           int ox, int lgrp, bool keep) {
+  constexpr int ES = DT == IFM_F64 ? 8 : (DT == IFM_F32 ? 4 : 2);
+  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+      (void *)((char *)a.out + T.obase * ES), (short)0, (int)(a.out_plane * ES), MF_RSRC_W3);
+  const bool colok = keep && ox < a.o0;
 #pragma unroll
   for (int b = 0; b < MF_B; ++b) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int oy = T.oy0 + b + MF_B * (4 * lgrp + i);
-      if (full || oy < a.o1) store_f(a.out, T.obase + (int64_t)oy * a.o0 + ox, DT, acc[b][i] * osc);
+      const uint32_t off = (colok && oy < a.o1) ? ((uint32_t)oy * (uint32_t)a.o0 + (uint32_t)ox) * ES : MF_DROP;
+      const float v = acc[b][i] * osc;
+      if constexpr (DT == IFM_F32) {
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), rs, off, 0, 0);
+      } else if constexpr (DT == IFM_F64) {
+        const uint64_t d = __builtin_bit_cast(uint64_t, (double)v);
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b64(rs, 0, 0, 0)), d), rs, off, 0, 0);
+      } else if constexpr (DT == IFM_BF16) {
+        __builtin_amdgcn_raw_buffer_store_b16(f32_to_bf16(v), rs, off, 0, 0);
+      } else {
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(uint16_t, (_Float16)v), rs, off, 0, 0);
+      }
     }
   }
 }
 
-__device__ __forceinline__ void mf_store(const MfmaArgs &a, const MfTile &T, const f32x4 (&acc)[MF_B], float osc) {
+__device__ __forceinline__ void mf_store(const MfmaArgs &a, const MfTile &T, const f32x4 (&acc)[MF_B], float osc,
+                                         bool keep) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int ox = T.ox0 + 16 * wave + (lane & 15);
-  if (ox >= a.o0) return;
   switch (mf_opaque(a.dt_out)) {
-  case IFM_F32: mf_store_t<IFM_F32>(a, T, acc, osc, ox, lane >> 4); break;
-  case IFM_F64: mf_store_t<IFM_F64>(a, T, acc, osc, ox, lane >> 4); break;
-  case IFM_BF16: mf_store_t<IFM_BF16>(a, T, acc, osc, ox, lane >> 4); break;
-  default: mf_store_t<IFM_F16>(a, T, acc, osc, ox, lane >> 4); break;
+  case IFM_F32: mf_store_t<IFM_F32>(a, T, acc, osc, ox, lane >> 4, keep); break;
+  case IFM_F64: mf_store_t<IFM_F64>(a, T, acc, osc, ox, lane >> 4, keep); break;
+  case IFM_BF16: mf_store_t<IFM_BF16>(a, T, acc, osc, ox, lane >> 4, keep); break;
+  default: mf_store_t<IFM_F16>(a, T, acc, osc, ox, lane >> 4, keep); break;
   }
 }
 
@@ -508,20 +583,9 @@ __global__ void __launch_bounds__(256, 2) k_dense_mfma(MfmaArgs a0, MfBatch bt) 
     __syncthreads();
     amax = fmaxf(fmaxf(red[par][0], red[par][1]), fmaxf(red[par][2], red[par][3]));
     const bool tilebad = redflag[par][0] | redflag[par][1] | redflag[par][2] | redflag[par][3];
-    if (tilebad) {
-      mf_direct(a, T);
-      if constexpr (G::NB == 1) {
-        if (t + 1 < t1) {
-          int ln, rn;
-          const MfmaArgs &an = mf_args<BATCH>(a0, bt, t + 1, ln, rn);
-          const MfTile Tn = mf_tile(an, ln);
-          mf_maps<G>(an, Tn, rmap, cmap);
-          __syncthreads();
-          mf_issue_dt<G, 0>(an, Tn, rmap, cmap, raw);
-        }
-      }
-      continue;
-    }
+    // A tile holding Inf or NaN is summed directly; it then follows the same
+    // path as the others with its matrix work skipped and its stores dropped.
+    if (tilebad) mf_direct(a, T);
     // Power-of-two scale putting the tile's largest |x| in [0.5, 1).
     int e = 0;
     if (amax > 0.f) frexpf(amax, &e);
@@ -529,14 +593,14 @@ __global__ void __launch_bounds__(256, 2) k_dense_mfma(MfmaArgs a0, MfBatch bt) 
     const float xs = ldexpf(1.0f, -e);
     const float xinv = ldexpf(1.0f, e);
     if constexpr (G::NB == 1) {
-      mf_put<G, 0>(lds32, raw, xs, true);
+      if (!tilebad) mf_put<G, 0>(lds32, raw, xs, true);
       if (t + 1 < t1) {
         int ln, rn;
         const MfmaArgs &an = mf_args<BATCH>(a0, bt, t + 1, ln, rn);
         mf_maps<G>(an, mf_tile(an, ln), rmap, cmap);
       }
     } else {
-      mf_convert<G>(lds32, xs);
+      if (!tilebad) mf_convert<G>(lds32, xs);
     }
     __syncthreads();
     if constexpr (G::NB == 1) {
@@ -557,22 +621,26 @@ __global__ void __launch_bounds__(256, 2) k_dense_mfma(MfmaArgs a0, MfBatch bt) 
     f32x4 acc[MF_B];
 #pragma unroll
     for (int b = 0; b < MF_B; ++b) acc[b] = f32x4{0.f, 0.f, 0.f, 0.f};
-    // chunk c = kb * NCH + ch covers K block kb and kernel rows 8ch .. 8ch+7
-    if constexpr (NC == 1) {
-      mf_chunk<G>(lw, ahi, alo, acc);
-    } else if constexpr (NC == 2) {
-      mf_chunk<G>(lw, ahi, alo, acc);
-      mf_chunk<G>(lw + (NCH == 2 ? G::P : 32), bhi, blo, acc);
-    } else {
-      for (int c = 0; c < NC; c += 2) {
-        const int c1 = c + 1;
-        mf_chunk<G>(lw + (c % NCH) * G::P + 32 * (c / NCH), ahi, alo, acc);
-        mf_tload(a.ttab, (c + 2) % NC, ahi, alo);
-        mf_chunk<G>(lw + (c1 % NCH) * G::P + 32 * (c1 / NCH), bhi, blo, acc);
-        mf_tload(a.ttab, (c + 3) % NC, bhi, blo);
+    if (!tilebad) {
+      // chunk c = kb * NCH + ch covers K block kb and kernel rows 8ch .. 8ch+7
+      if constexpr (NC == 1) {
+        mf_chunk<G>(lw, ahi, alo, acc);
+      } else if constexpr (NC == 2 && NCH == 2) {
+        mf_rows16<G>(lw, ahi, alo, bhi, blo, acc);
+      } else if constexpr (NC == 2) {
+        mf_chunk<G>(lw, ahi, alo, acc);
+        mf_chunk<G>(lw + 32, bhi, blo, acc);
+      } else {
+        for (int c = 0; c < NC; c += 2) {
+          const int c1 = c + 1;
+          mf_chunk<G>(lw + (c % NCH) * G::P + 32 * (c / NCH), ahi, alo, acc);
+          mf_tload(a.ttab, (c + 2) % NC, ahi, alo);
+          mf_chunk<G>(lw + (c1 % NCH) * G::P + 32 * (c1 / NCH), bhi, blo, acc);
+          mf_tload(a.ttab, (c + 3) % NC, bhi, blo);
+        }
       }
     }
-    mf_store(a, T, acc, xinv * a.kinv);
+    mf_store(a, T, acc, xinv * a.kinv, !tilebad);
   }
 }
 
@@ -620,11 +688,12 @@ static int device_attr(hipDeviceAttribute_t attr, int dflt) {
   return x;
 }
 
-bool dense_mfma_supported(const DenseKernel &k, const Dense4 &in) {
+bool dense_mfma_supported(const DenseKernel &k, const Dense4 &in, const Dense4 &out) {
   const bool twod = k.n[2] == 1 && k.n[3] == 1;
   if (!(twod && mfma_kb(k.n[0]) <= MF_MAXKB && k.n[1] <= MF_MAXCH * MF_UCH)) return false;
   // buffer descriptors address one plane with 31-bit byte offsets
   if ((double)in.n[0] * (double)in.n[1] * (double)dtype_size(in.dtype) >= 2147483648.0) return false;
+  if ((double)out.n[0] * (double)out.n[1] * (double)dtype_size(out.dtype) >= 2147483648.0) return false;
   MfVariant v;
   if (!mf_select(mfma_kb(k.n[0]), mfma_nch(k.n[1]), &v)) return false;
   static int max_lds = -1;

@@ -311,6 +311,12 @@ __device__ __forceinline__ void mf_tload(const uint16_t *ttab, int c, f16x8 (&th
   }
 }
 
+// Waits for every outstanding vector-memory operation of the wave.  Used
+// right after the Toeplitz fragments are loaded: the sweep then carries no
+// wait for them, which (the counter being in order) would also wait for the
+// stores and loads issued after them.
+__device__ __forceinline__ void mf_wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }  // vmcnt(0)
+
 // One chunk: 8 kernel rows against the 8 interleaved blocks.  lbase points at
 // the lane's first word of LDS row (chunk row offset + lane row) and column
 // (K block + wave + lane group).
@@ -538,6 +544,7 @@ __global__ void __launch_bounds__(256, 2) k_dense_mfma(MfmaArgs a0, MfBatch bt) 
   f16x8 ahi[MF_UCH], alo[MF_UCH], bhi[MF_UCH], blo[MF_UCH];
   mf_tload(af.ttab, 0, ahi, alo);
   if constexpr (NC > 1) mf_tload(af.ttab, 1, bhi, blo);
+  mf_wait_vmem();
 
   // Border maps: with one staging batch they are written between the two
   // barriers of the previous tile and read by the loads issued after the
@@ -614,6 +621,7 @@ __global__ void __launch_bounds__(256, 2) k_dense_mfma(MfmaArgs a0, MfBatch bt) 
       if (rt != cur) {  // another request: its Toeplitz fragments
         mf_tload(a.ttab, 0, ahi, alo);
         if constexpr (NC > 1) mf_tload(a.ttab, 1, bhi, blo);
+        mf_wait_vmem();
         cur = rt;
       }
     }

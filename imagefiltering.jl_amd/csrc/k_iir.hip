@@ -125,6 +125,132 @@ __global__ void __launch_bounds__(256) k_iir_strided(IirArgs p) {
   }
 }
 
+// Lines along the contiguous axis.  One thread per line would read 64 lines
+// per wave instruction, n elements apart; instead a workgroup owns 256 lines
+// and walks them in chunks of IR_C elements staged through LDS: the chunk is
+// loaded and stored with consecutive lanes on consecutive elements
+// (coalesced), and each thread runs its line's recursion over the chunk from
+// LDS.  The next chunk's loads are issued before the current chunk is
+// filtered.  The arithmetic per element is that of k_iir_strided.
+constexpr int IR_C = 16;                   // chunk length
+constexpr int IR_L = 256;                  // lines per workgroup
+constexpr int IR_P = IR_C + 1;             // LDS pitch (doubles)
+constexpr int IR_E = IR_L * IR_C / 256;    // staged elements per thread
+
+template <int DT>
+__global__ void __launch_bounds__(256, 2) k_iir_rows(IirArgs p) {
+  extern __shared__ double tile[];  // IR_L x IR_P
+  const int t = threadIdx.x;
+  const int64_t line0 = (int64_t)blockIdx.x * IR_L;
+  const int n = p.n[0];
+  const int nl = (int)min((int64_t)IR_L, p.lines - line0);
+  const int nch = (n + IR_C - 1) / IR_C;
+  const double a1 = p.a[0], a2 = p.a[1], a3 = p.a[2];
+  const double b1 = p.b[0], b2 = p.b[1], b3 = p.b[2];
+  const double asum = a1 + a2 + a3, bsum = b1 + b2 + b3;
+  double reg[IR_E];
+  // staged element j of the thread: line (t + 256 j) / IR_C, offset (t + 256 j) % IR_C
+  auto fetch = [&](const void *src, int dtsrc, int c0, bool typed) {
+#pragma unroll
+    for (int j = 0; j < IR_E; ++j) {
+      const int q = t + 256 * j, l = q / IR_C, e = q % IR_C;
+      const int i = c0 + e;
+      if (l < nl && i < n) {
+        const int64_t g = (line0 + l) * n + i;
+        reg[j] = typed ? iir_ld<DT>(p, g) : ((const double *)src)[g];
+      }
+    }
+    (void)dtsrc;
+  };
+  auto put = [&]() {
+#pragma unroll
+    for (int j = 0; j < IR_E; ++j) {
+      const int q = t + 256 * j;
+      tile[(q / IR_C) * IR_P + q % IR_C] = reg[j];
+    }
+  };
+  auto store = [&](int c0) {
+#pragma unroll
+    for (int j = 0; j < IR_E; ++j) {
+      const int q = t + 256 * j, l = q / IR_C, e = q % IR_C;
+      const int i = c0 + e;
+      if (l < nl && i < n) p.dst[(line0 + l) * n + i] = tile[l * IR_P + e];
+    }
+  };
+  double *row = tile + t * IR_P;
+  // forward: y[i] = x[i] + a1 y[i-1] + a2 y[i-2] + a3 y[i-3], with y[-k] = u-
+  double um = 0.0, ym1 = 0.0, ym2 = 0.0, ym3 = 0.0, iplus = p.fill;
+  fetch(nullptr, 0, 0, true);
+  for (int c = 0; c < nch; ++c) {
+    const int c0 = c * IR_C;
+    put();
+    __syncthreads();
+    if (c + 1 < nch) fetch(nullptr, 0, c0 + IR_C, true);
+    if (t < nl) {
+      const int e1 = min(IR_C, n - c0);
+      if (c == 0) {
+        const double iminus = p.isfill ? p.fill : row[0];
+        um = iminus / (1 - asum);
+      }
+      if (c0 + e1 == n && !p.isfill) iplus = row[e1 - 1];
+      for (int e = 0; e < e1; ++e) {
+        const int i = c0 + e;  // wave-uniform
+        const double x = row[e];
+        double y;
+        if (i == 0) y = x + a1 * um + a2 * um + a3 * um;
+        else if (i == 1) y = x + a1 * ym1 + a2 * um + a3 * um;
+        else if (i == 2) y = x + a1 * ym1 + a2 * ym2 + a3 * um;
+        else y = x + a1 * ym1 + a2 * ym2 + a3 * ym3;
+        row[e] = y;
+        ym3 = ym2;
+        ym2 = ym1;
+        ym1 = y;
+      }
+    }
+    __syncthreads();
+    store(c0);
+    __syncthreads();
+  }
+  // right edge (ym1 = y[n-1], ym2 = y[n-2], ym3 = y[n-3]), then backward:
+  // z[k] = y[k] + b1 z[k+1] + b2 z[k+2] + b3 z[k+3], the last three from M
+  const double sc = p.scale;
+  const double uplus = iplus / (1 - asum);
+  const double vplus = uplus / (1 - bsum);
+  const double d0 = ym1 - uplus, d1 = ym2 - uplus, d2 = ym3 - uplus;
+  const double v0 = p.M[0] * d0 + p.M[1] * d1 + p.M[2] * d2 + vplus;
+  const double v1 = p.M[3] * d0 + p.M[4] * d1 + p.M[5] * d2 + vplus;
+  const double v2 = p.M[6] * d0 + p.M[7] * d1 + p.M[8] * d2 + vplus;
+  const double z2 = v0;
+  const double z1 = ym2 + b1 * z2 + b2 * v1 + b3 * v2;
+  const double z0 = ym3 + b1 * z1 + b2 * z2 + b3 * v1;
+  double zp1 = 0.0, zp2 = 0.0, zp3 = 0.0;
+  fetch(p.dst, IFM_F64, (nch - 1) * IR_C, false);
+  for (int c = nch - 1; c >= 0; --c) {
+    const int c0 = c * IR_C;
+    put();
+    __syncthreads();
+    if (c > 0) fetch(p.dst, IFM_F64, c0 - IR_C, false);
+    if (t < nl) {
+      const int e1 = min(IR_C, n - c0);
+      for (int e = e1 - 1; e >= 0; --e) {
+        const int k = c0 + e;  // wave-uniform
+        double z;
+        if (k == n - 1) z = z2;
+        else if (k == n - 2) z = z1;
+        else if (k == n - 3) z = z0;
+        else z = row[e] + b1 * zp1 + b2 * zp2 + b3 * zp3;
+        row[e] = z * sc;
+        zp3 = zp2;
+        zp2 = zp1;
+        zp1 = z;
+      }
+    }
+    __syncthreads();
+    store(c0);
+    __syncthreads();
+  }
+}
+
 hipError_t launch_iir(const Dense4 &in, Dense4 &out, int axis, const IirCoef &c, int isfill, double fill,
                       hipStream_t st) {
   IirArgs p;
@@ -148,6 +274,14 @@ hipError_t launch_iir(const Dense4 &in, Dense4 &out, int axis, const IirCoef &c,
   p.isfill = isfill;
   p.fill = fill;
   if (p.lines == 0) return hipSuccess;
+  if (stride == 1) {  // lines along the contiguous axis: LDS-staged chunks
+    const unsigned nb = (unsigned)((p.lines + IR_L - 1) / IR_L);
+    const size_t lds = sizeof(double) * IR_L * IR_P;
+    if (in.dtype == IFM_F32) hipLaunchKernelGGL(k_iir_rows<IFM_F32>, dim3(nb), dim3(256), lds, st, p);
+    else if (in.dtype == IFM_F64) hipLaunchKernelGGL(k_iir_rows<IFM_F64>, dim3(nb), dim3(256), lds, st, p);
+    else hipLaunchKernelGGL(k_iir_rows<-1>, dim3(nb), dim3(256), lds, st, p);
+    return hipGetLastError();
+  }
   int64_t blocks = (p.lines + 255) / 256;
   const bool s1 = stride == 1;
   if (in.dtype == IFM_F32) {

@@ -442,6 +442,24 @@ def test_iir_matches_oracle(border):
     np.testing.assert_allclose(out, want, rtol=1e-9, atol=1e-12)
 
 
+def test_iir_contiguous_lines_chunk_edges():
+    # lines along the contiguous axis whose last three points straddle a chunk
+    # boundary (n = 17, 18), the shortest accepted line (n = 4), a partial last
+    # group of lines, F32 / F64 input, Pad and Fill borders
+    rng = np.random.default_rng(23)
+    for shape, dt in [((17, 300), np.float32), ((18, 5, 4), np.float64), ((4, 5), np.float32),
+                      ((33, 7, 4), np.float32), ((35, 260), np.float64)]:
+        x = rng.random(shape)
+        x = x.astype(dt)
+        x = np.asfortranarray(x)
+        kern = KernelFactors.IIRGaussian(tuple(1.5 for _ in shape))
+        for border in ["replicate", IF.Fill(0.25)]:
+            out = IF.imfilter(x, kern, border)
+            ob = ("fill", border.value) if isinstance(border, IF.Fill) else "replicate"
+            want = O.imfilter_iir(x.astype(np.float64), iir_spec(kern), ob)
+            np.testing.assert_allclose(out, want, rtol=1e-9, atol=1e-12, err_msg="%s %s" % (shape, border))
+
+
 def test_iir_per_dim_call_and_errors():
     imgf = np.zeros((5, 7))
     imgf[2, 3] = 1

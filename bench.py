@@ -243,6 +243,21 @@ def run_extras(IF, _lib, L, ctx, T, img, d_img, d_out, b_reflect, args):
     a_b, a_ob = ifm_array(xb), ifm_array(ob)
     ks15 = _kernel_struct(k15)
     rate("dense15_mfma_bf16io_4096", lambda: L.ifm_imfilter_dense(ctx, a_b, a_ob, None, ks15, b_reflect, 0), n * n, 4)
+    # the same 15 x 15 call on host arrays: upload, filter, download (wall clock, PCIe included)
+    out_h = np.empty_like(img)
+    ah_in, ah_out = ifm_array(img), ifm_array(out_h)
+    for _ in range(2):
+        _lib.check(L.ifm_imfilter_dense(ctx, ah_in, ah_out, None, ks15, b_reflect, 0))
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        L.ifm_imfilter_dense(ctx, ah_in, ah_out, None, ks15, b_reflect, 0)
+    ms_h = (time.perf_counter() - t0) * 1e3 / reps
+    ex["dense15_mfma_4096_host_arrays"] = {"ms": round(ms_h, 4), "gpix_s": round(n * n / (ms_h * 1e-3) / 1e9, 3),
+                                           "note": "host numpy arrays in and out; wall clock incl. PCIe copies"}
+    log("extra %-28s %8.3f ms  %8.2f Gpix/s  (host arrays, PCIe included)" % ("dense15_mfma_4096_host_arrays", ms_h,
+                                                                             n * n / (ms_h * 1e-3) / 1e9))
+    del out_h
     # config 1: 1024^2 gaussian sigma 3 replicate, F32 -> F64
     c1 = IF.DeviceArray.from_host(np.random.default_rng(1).random((1024, 1024), dtype=np.float32))
     o1 = c1.similar(IF.Float64)

@@ -299,6 +299,26 @@ def test_dense_mfma_many_tiles_per_workgroup(K, dtype):
         assert e <= TOL, (K, dtype, b, e)
 
 
+@pytest.mark.parametrize("K", [15, 17])
+def test_dense_mfma_interior_tiles_2d(K):
+    """Column-major 2-D images with many interior tiles (staged by 16-byte
+    quads) and border tiles (staged element by element); widths that are and
+    are not multiples of 4 (vector stores or element stores); F32 and F64
+    outputs; every border."""
+    rng = np.random.default_rng(60 + K)
+    k = norm_random_kernel(rng, K)
+    kern = IF.centered(k)
+    for shape in [(700, 520), (701, 523)]:
+        x = np.asfortranarray(rng.random(shape).astype(np.float32))
+        for b in ["reflect", "circular", IF.Fill(0.25)]:
+            for T in (None, IF.Float64):
+                out = IF.imfilter(x, kern, b, IF.Algorithm.FIR()) if T is None else \
+                    IF.imfilter(T, x, kern, b, IF.Algorithm.FIR())
+                want = O.imfilter_dense(x.astype(np.float64), k, list(kern.first), obord(b))
+                e = relerr(out, want)
+                assert e <= TOL, (K, shape, b, T, e)
+
+
 def _dense_lib(img, k, first, border, precision):
     """ifm_imfilter_dense with an explicit precision on device arrays."""
     from imagefiltering import _lib

@@ -64,7 +64,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the other configurations (serving runs unless --serve-requests 0)")
     ap.add_argument("--serve-requests", type=int, default=20000)
     ap.add_argument("--profile-copy", action="store_true",
                     help="also run one plain copy of a known size (rocprof counter calibration)")
@@ -133,10 +133,10 @@ def main():
 
     extras = {}
     serving = None
+    del out
     if not args.no_extras:
-        del out
         extras = run_extras(IF, _lib, L, ctx, T, img, d_img, d_out, b_reflect, args)
-        serving = run_serving(args)
+    serving = run_serving(args)  # --serve-requests 0 skips it
     if args.profile_copy:
         # a plain f32 copy of a known size: 256 MiB read + 256 MiB written
         m = 1 << 26

@@ -134,6 +134,19 @@ def main():
     extras = {}
     serving = None
     del out
+    tile_tune = None
+    if not args.no_extras or args.serve_requests > 0:
+        # tile shapes of the fused separable kernel for this machine (the C1 extra
+        # and the serving stream's separable requests use them)
+        from imagefiltering import tiles
+        t0 = time.time()
+        tt = tiles.tune()
+        tiles.save(tt)
+        tile_tune = {"seconds": round(time.time() - t0, 2),
+                     "rules": [[r["dtype"], r["kmax"], r["max_pixels"], r["tile_h"]] for r in tt["rules"]],
+                     "rule_fields": ["dtype", "kmax", "max_pixels", "tile_h"], "trials": tt["trials"]}
+        log("bench: tile autotune %.1f s: %s" % (tile_tune["seconds"],
+                                                 ["%s k<=%d px<=%d -> %d" % tuple(r) for r in tile_tune["rules"]]))
     if not args.no_extras:
         extras = run_extras(IF, _lib, L, ctx, T, img, d_img, d_out, b_reflect, args)
     serving = run_serving(args)  # --serve-requests 0 skips it
@@ -180,6 +193,7 @@ def main():
                                    % (r0, r0 + nrows - 1, nrows)},
         "serving": serving,
         "extras": extras,
+        "tile_autotune": tile_tune,
     }
     print(json.dumps(rec), flush=True)
 
@@ -188,7 +202,9 @@ def run_serving(args):
     """The serving mix: the same seeded request stream through the batched
     request loop and through plain per-call dispatch, first closed-loop (all
     requests arrive at once: saturation throughput), then open-loop with
-    Poisson arrivals at 0.8 x the batched saturation rate (latency)."""
+    Poisson arrivals at 0.8 x the batched saturation rate (latency).  The
+    batch size and in-flight depth come from the schedule autotuner, run on a
+    sample of the same stream first."""
     from imagefiltering import serving
     n = args.serve_requests
     if n <= 0:
@@ -198,8 +214,12 @@ def run_serving(args):
            "mix": "sides log-uniform [64,4096]; F32 or BF16; gaussian sigma~U[0.5,8] | sobel gradient pair | "
                   "median 7x7 | dense 15x15; five borders; inputs generated on the device"}
     mix = serving.request_mix(n, rate=None)
+    sched, table = serving.autotune(mix, budget_bytes=budget)
+    res["schedule"] = sched
+    res["autotune_trials"] = table
+    log("serving autotune: %s" % sched)
     for mode in ("batched", "percall"):
-        _, done = serving.serve(mix, mode, budget_bytes=budget)
+        _, done = serving.serve(mix, mode, budget_bytes=budget, **sched)
         res["closed_" + mode] = serving.stats(mix, done)
         log("serving closed-loop %-8s %s" % (mode, res["closed_" + mode]))
     sat = res["closed_batched"]["images_per_s"]
@@ -207,7 +227,7 @@ def run_serving(args):
     mix = serving.request_mix(n, rate=rate)
     res["open_rate_per_s"] = rate
     for mode in ("batched", "percall"):
-        _, done = serving.serve(mix, mode, budget_bytes=budget)
+        _, done = serving.serve(mix, mode, budget_bytes=budget, **sched)
         res["open_" + mode] = serving.stats(mix, done)
         log("serving open-loop %.0f/s %-8s %s" % (rate, mode, res["open_" + mode]))
     res["speedup_images_per_s"] = sat / res["closed_percall"]["images_per_s"]

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "ifm_internal.h"
+#include "k_tiles.h"
 
 using namespace ifm;
 
@@ -191,6 +192,28 @@ bool is_float_out(int dt) { return dt == IFM_F32 || dt == IFM_F64 || dt == IFM_B
 
 }  // namespace
 
+namespace ifm {
+
+int sep2d_tile_h(const ifm_ctx *ctx, int dt_in, int k0, int k1, int64_t pixels) {
+  const int kmax = std::max(k0, k1);
+  for (const ifm_tile_rule &r : ctx->tile_rules) {
+    if (r.op != IFM_TILE_SEP2D || (r.dtype >= 0 && r.dtype != dt_in) || kmax > r.kmax || pixels > r.max_pixels)
+      continue;
+    if (r.tile_h == S2_TY) return S2_TY;
+    static int max_lds = -1;
+    if (max_lds < 0) {
+      int dev = 0;
+      if (hipGetDevice(&dev) != hipSuccess ||
+          hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+        max_lds = 64 * 1024;
+    }
+    return sizeof(float) * sep2d_lds_floats(k0, k1, S2_TY2) <= (size_t)max_lds ? S2_TY2 : S2_TY;
+  }
+  return S2_TY;
+}
+
+}  // namespace ifm
+
 // ---------------------------------------------------------------------------
 extern "C" {
 
@@ -280,6 +303,26 @@ int ifm_memset_d(ifm_ctx *ctx, void *dst, int value, uint64_t bytes) {
   if (!ctx) return IFM_E_ARG;
   if (bytes) HIPCHK(hipMemsetAsync(dst, value, bytes, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
+  return IFM_OK;
+}
+
+int ifm_set_tile_rules(ifm_ctx *ctx, const ifm_tile_rule *rules, int n) {
+  if (!ctx) return IFM_E_ARG;
+  if (n < 0 || n > IFM_MAX_TILE_RULES || (n > 0 && !rules)) return fail(ctx, IFM_E_ARG, "bad tile rule count");
+  for (int i = 0; i < n; ++i) {
+    const ifm_tile_rule &r = rules[i];
+    if (r.op != IFM_TILE_SEP2D) return fail(ctx, IFM_E_ARG, "tile rule: unknown op");
+    if (r.tile_h != S2_TY && r.tile_h != S2_TY2) return fail(ctx, IFM_E_ARG, "tile rule: tile_h must be 32 or 64");
+    if (r.dtype < -1 || r.dtype > IFM_F16) return fail(ctx, IFM_E_ARG, "tile rule: bad dtype");
+  }
+  ctx->tile_rules.assign(rules, rules + n);
+  return IFM_OK;
+}
+
+int ifm_tile_height(ifm_ctx *ctx, int op, int dtype, int kmax, int64_t pixels, int *tile_h) {
+  if (!ctx || !tile_h) return IFM_E_ARG;
+  if (op != IFM_TILE_SEP2D) return fail(ctx, IFM_E_ARG, "unknown tile op");
+  *tile_h = sep2d_tile_h(ctx, dtype, kmax, kmax, pixels);
   return IFM_OK;
 }
 
@@ -446,7 +489,8 @@ int ifm_imfilter_fir(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const in
         if (groups[gi].size() == 2) {
           // sep2d reads planes p2+f2, p3+f3 of its source; the source is the
           // input here (first group), so f2/f3 are the destination offsets.
-          HIPCHK(launch_sep2d(src, dst, dst_first, pass_of(0), pass_of(1), style, (float)fill, st));
+          const int ty = sep2d_tile_h(ctx, src.dtype, pass_of(0).len, pass_of(1).len, dst.n[0] * dst.n[1]);
+          HIPCHK(launch_sep2d(src, dst, dst_first, pass_of(0), pass_of(1), style, (float)fill, ty, st));
           fill *= tapsum(0) * tapsum(1);
         } else {
           int m = groups[gi][0];

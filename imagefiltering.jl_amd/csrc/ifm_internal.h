@@ -178,7 +178,7 @@ hipError_t launch_pad(const Dense4 &in, Dense4 &out, const int64_t *lo, int styl
 hipError_t launch_fir1d(const Dense4 &in, const int64_t *in_first, Dense4 &out, const int64_t *out_first,
                         const int64_t *img_n, const FirPass &p, int style, float fill, hipStream_t st);
 hipError_t launch_sep2d(const Dense4 &in, Dense4 &out, const int64_t *out_first, const FirPass &p0,
-                        const FirPass &p1, int style, float fill, hipStream_t st);
+                        const FirPass &p1, int style, float fill, int ty, hipStream_t st);
 // Three factors on axes 0, 1, 2 in one pass (input tile <= 4096 elements,
 // axis-2 factor odd and <= 9 taps, F32 or F64 input, an (x, y) plane < 4 GiB).
 bool sep3d_supported(int k0, int k1, int k2, int dt_in, int64_t plane_elems);
@@ -296,6 +296,7 @@ struct SepReq {
   int k0, k1, first0, first1;
   int style, dt_in, dt_out;
   float fill;
+  int ty;  // tile height (S2_TY or S2_TY2)
 };
 hipError_t launch_sep_batch(const SepReq *reqs, const int *tile_req, const int *tile_first, int ntiles,
                             size_t lds_bytes, hipStream_t st);
@@ -336,9 +337,13 @@ struct ifm_ctx {
   };
   std::vector<BatchSlot> bslots;
   int bnext = 0;
+  std::vector<ifm_tile_rule> tile_rules;  // ifm_set_tile_rules
 };
 
 namespace ifm {
 void batch_release(ifm_ctx *ctx);
+// Tile height of the fused two-factor kernel for a call (the first matching
+// tile rule, else S2_TY; tiles whose LDS would not fit fall back to S2_TY).
+int sep2d_tile_h(const ifm_ctx *ctx, int dt_in, int k0, int k1, int64_t pixels);
 
 }  // namespace ifm

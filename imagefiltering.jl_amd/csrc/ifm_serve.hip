@@ -275,7 +275,7 @@ int build_batch(ifm_ctx *ctx, const ifm_request *reqs, const int *idx, int n, co
     std::vector<int> mem, nt;
     std::vector<SepReq> sr;
     std::vector<std::pair<size_t, size_t>> tapoffs;  // (offset of taps0, taps1)
-    int maxk0 = 1, maxk1 = 1;
+    size_t lds = 0;
     for (int i = 0; i < n; ++i)
     for (int part = 0; part < 2; ++part) {
       const ifm_request &q = reqs[idx[i]];
@@ -310,12 +310,12 @@ int build_batch(ifm_ctx *ctx, const ifm_request *reqs, const int *idx, int n, co
       s.dt_in = vin[i].dtype;
       s.dt_out = part ? q.out2.dtype : vout[i].dtype;
       s.fill = (float)q.border.fill;
+      s.ty = sep2d_tile_h(ctx, s.dt_in, k0, k1, (int64_t)s.n0 * s.n1);
       sr.push_back(s);
       tapoffs.push_back({blk.put(t0, sizeof(float) * k0, 16), blk.put(t1, sizeof(float) * k1, 16)});
-      maxk0 = std::max(maxk0, k0);
-      maxk1 = std::max(maxk1, k1);
+      lds = std::max(lds, sizeof(float) * sep2d_lds_floats(k0, k1, s.ty));
       mem.push_back(i);
-      nt.push_back((int)(((s.n0 + S2_TX - 1) / S2_TX) * ((s.n1 + S2_TY - 1) / S2_TY)));
+      nt.push_back((int)(((s.n0 + S2_TX - 1) / S2_TX) * ((s.n1 + s.ty - 1) / s.ty)));
     }
     if (!sr.empty()) {
       size_t off = blk.put(sr.data(), sr.size() * sizeof(SepReq));
@@ -323,7 +323,7 @@ int build_batch(ifm_ctx *ctx, const ifm_request *reqs, const int *idx, int n, co
         blk.ptr(off + m * sizeof(SepReq) + offsetof(SepReq, taps0), tapoffs[m].first);
         blk.ptr(off + m * sizeof(SepReq) + offsetof(SepReq, taps1), tapoffs[m].second);
       }
-      tiles_group(1, 0, mem, nt, off, sizeof(float) * sep2d_lds_floats(maxk0, maxk1));
+      tiles_group(1, 0, mem, nt, off, lds);
     }
   }
   // Median, per (window, key width); small or F64 images run alone, and the

@@ -223,7 +223,7 @@ hipError_t launch_fir1d(const Dense4 &in, const int64_t *in_first, Dense4 &out, 
 
 // ---------------------------------------------------------------------------
 // sep2d: fused pair of factors on canonical axes 0 and 1, batched over axes
-// 2 and 3.  An output tile of 64 x 32 pixels loads its halo'd input tile into
+// 2 and 3.  An output tile of 64 x 32 or 64 x 64 pixels loads its halo'd input tile into
 // LDS once (border map on load), runs the axis-0 factor into a second LDS
 // tile and the axis-1 factor from there, so HBM sees one read and one write.
 // ---------------------------------------------------------------------------
@@ -241,6 +241,7 @@ struct Sep2dArgs {
   int style;
   int dt_in, dt_out;
   float fill;
+  int ty;              // tile height (S2_TY or S2_TY2)
   float taps0[S2_MAXK];
   float taps1[S2_MAXK];
 };
@@ -262,8 +263,9 @@ __global__ void __launch_bounds__(256) k_sep2d(Sep2dArgs a) {
   t.dt_in = a.dt_in;
   t.dt_out = a.dt_out;
   t.fill = a.fill;
+  t.ty = a.ty;
   t.ox0 = blockIdx.x * S2_TX;
-  t.oy0 = blockIdx.y * S2_TY;
+  t.oy0 = blockIdx.y * a.ty;
   t.x0 = a.f0 + t.ox0;
   t.y0 = a.f1 + t.oy0;
   for (int plane = blockIdx.z; plane < a.nplanes; plane += gridDim.z) {
@@ -331,6 +333,7 @@ __global__ void __launch_bounds__(256) k_sep3d(Sep3dArgs a) {
   t.dt_in = a.dt_in;
   t.dt_out = a.dt_out;
   t.fill = a.fill;
+  t.ty = S2_TY;
   t.ox0 = blockIdx.x * S2_TX;
   t.oy0 = blockIdx.y * S2_TY;
   t.x0 = a.f0 + t.ox0;
@@ -492,7 +495,7 @@ hipError_t launch_sep3d(const Dense4 &in, Dense4 &out, const int64_t *out_first,
 }
 
 // Batched serving: tile g of the launch is tile g - tile_first[r] of request
-// r = tile_req[g] (row-major over its 64 x 32 tiles).
+// r = tile_req[g] (row-major over its 64 x ty tiles).
 __global__ void __launch_bounds__(256) k_sep_batch(const SepReq *__restrict__ reqs, const int *__restrict__ tile_req,
                                                    const int *__restrict__ tile_first) {
   extern __shared__ float smem[];
@@ -510,8 +513,9 @@ __global__ void __launch_bounds__(256) k_sep_batch(const SepReq *__restrict__ re
   t.n1 = q.n1;
   t.o0 = q.n0;
   t.o1 = q.n1;
+  t.ty = q.ty;
   t.ox0 = (local % ntx) * S2_TX;
-  t.oy0 = (local / ntx) * S2_TY;
+  t.oy0 = (local / ntx) * q.ty;
   t.x0 = t.ox0;
   t.y0 = t.oy0;
   t.k0 = q.k0;
@@ -533,7 +537,8 @@ hipError_t launch_sep_batch(const SepReq *reqs, const int *tile_req, const int *
 }
 
 hipError_t launch_sep2d(const Dense4 &in, Dense4 &out, const int64_t *out_first, const FirPass &p0,
-                        const FirPass &p1, int style, float fill, hipStream_t st) {
+                        const FirPass &p1, int style, float fill, int ty, hipStream_t st) {
+  if (ty != S2_TY && ty != S2_TY2) return hipErrorInvalidValue;
   Sep2dArgs a;
   a.in = in.data;
   a.out = out.data;
@@ -561,8 +566,9 @@ hipError_t launch_sep2d(const Dense4 &in, Dense4 &out, const int64_t *out_first,
   for (int t = 0; t < p0.len; ++t) a.taps0[t] = p0.taps[t];
   for (int t = 0; t < p1.len; ++t) a.taps1[t] = p1.taps[t];
   if (out.count() == 0) return hipSuccess;
-  const size_t lds = sizeof(float) * sep2d_lds_floats(a.k0, a.k1);
-  dim3 grid((a.o0 + S2_TX - 1) / S2_TX, (a.o1 + S2_TY - 1) / S2_TY, a.nplanes < 65535 ? a.nplanes : 65535);
+  a.ty = ty;
+  const size_t lds = sizeof(float) * sep2d_lds_floats(a.k0, a.k1, ty);
+  dim3 grid((a.o0 + S2_TX - 1) / S2_TX, (a.o1 + ty - 1) / ty, a.nplanes < 65535 ? a.nplanes : 65535);
   hipLaunchKernelGGL(k_sep2d, grid, dim3(256), lds, st, a);
   return hipGetLastError();
 }

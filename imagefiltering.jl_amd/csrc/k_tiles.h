@@ -8,18 +8,22 @@ namespace ifm {
 // ---------------------------------------------------------------------------
 // Fused pair of 1-D factors on axes 0 and 1 (the two-factor cascade of
 // src/imfilter.jl:372-457 with the factor loop of :671-739).  One workgroup of
-// 256 threads computes a 64 x 32 output tile:
-//   1. the (32 + k1 - 1) x (64 + k0 - 1) input tile is loaded once into LDS,
+// 256 threads computes a 64 x TY output tile, TY = 32 or 64 (the tile shape is
+// a launch choice: a taller tile re-reads fewer halo rows per output, a
+// shorter one makes more workgroups for small images):
+//   1. the (TY + k1 - 1) x (64 + k0 - 1) input tile is loaded once into LDS,
 //      border map on load; each thread owns one column and every other row.
 //      Loads keep the raw element bits and are issued in groups of 8 before
 //      any is used (issue / commit), so several are in flight per thread;
 //   2. the axis-0 factor runs from registers: a thread slides over k0 + 3
 //      LDS values to produce 4 consecutive outputs of one row (mid tile);
-//   3. the axis-1 factor likewise produces 8 consecutive rows of one column.
+//   3. the axis-1 factor likewise produces 8 consecutive rows of one column,
+//      once per 32-row band of the tile.
 // Accumulation is FP32 in tap order, like the reference's loops.
 // ---------------------------------------------------------------------------
 constexpr int S2_TX = 64;
-constexpr int S2_TY = 32;
+constexpr int S2_TY = 32;   // the default tile height; S2_TY2 is the tall one
+constexpr int S2_TY2 = 64;
 constexpr int S2_MAXK = 64;
 
 struct Sep2dTile {
@@ -33,19 +37,20 @@ struct Sep2dTile {
   int k0, k1, first0, first1;
   int style, dt_in, dt_out;
   float fill;
+  int ty;                // output rows of the tile: S2_TY or S2_TY2
 };
 
 // LDS: input tile (odd pitch), the mid tile at a 16-byte boundary, then the
 // row table (border-mapped input row of each tile row, -1 for Fill).
-__host__ __device__ inline int sep2d_mid_offset(int k0, int k1) {
-  const int in_h = S2_TY + k1 - 1, in_w = S2_TX + k0 - 1;
+__host__ __device__ inline int sep2d_mid_offset(int k0, int k1, int ty = S2_TY) {
+  const int in_h = ty + k1 - 1, in_w = S2_TX + k0 - 1;
   return (in_h * (in_w | 1) + 3) & ~3;
 }
-__host__ __device__ inline int sep2d_rows_offset(int k0, int k1) {
-  return sep2d_mid_offset(k0, k1) + (S2_TY + k1 - 1) * S2_TX;
+__host__ __device__ inline int sep2d_rows_offset(int k0, int k1, int ty = S2_TY) {
+  return sep2d_mid_offset(k0, k1, ty) + (ty + k1 - 1) * S2_TX;
 }
-__host__ __device__ inline size_t sep2d_lds_floats(int k0, int k1) {
-  return (size_t)sep2d_rows_offset(k0, k1) + (size_t)(S2_TY + k1 - 1);
+__host__ __device__ inline size_t sep2d_lds_floats(int k0, int k1, int ty = S2_TY) {
+  return (size_t)sep2d_rows_offset(k0, k1, ty) + (size_t)(ty + k1 - 1);
 }
 
 // Raw element bits (F64 is narrowed at once) and their value: a load is not
@@ -83,7 +88,7 @@ __device__ __forceinline__ StageLane stage_lane(const Sep2dTile &a, int in_w, in
 
 // Fills the row table of the tile (the caller owns the barrier after it).
 __device__ __forceinline__ void sep2d_rows(const Sep2dTile &a, int *s_rows) {
-  const int in_h = S2_TY + a.k1 - 1;
+  const int in_h = a.ty + a.k1 - 1;
   for (int i = threadIdx.x; i < in_h; i += blockDim.x) s_rows[i] = bmap(a.style, a.y0 + a.first1 + i, a.n1);
 }
 
@@ -134,9 +139,10 @@ __device__ __forceinline__ void sep2d_stage(const Sep2dTile &a, const int *s_row
   }
 }
 
-__device__ __forceinline__ void sep2d_store(const Sep2dTile &a, const float (&acc)[8]) {
+// Stores the thread's 8 outputs of the 32-row band starting at tile row h.
+__device__ __forceinline__ void sep2d_store(const Sep2dTile &a, const float (&acc)[8], int h = 0) {
   const int tid = threadIdx.x;
-  const int col = tid & 63, r0 = (tid >> 6) * 8;
+  const int col = tid & 63, r0 = h + (tid >> 6) * 8;
   const int ox = a.ox0 + col;
   if (ox < a.o0) {
 #pragma unroll
@@ -147,15 +153,14 @@ __device__ __forceinline__ void sep2d_store(const Sep2dTile &a, const float (&ac
   }
 }
 
-// The two factor passes over a staged tile (after the staging barrier); on
-// return thread t holds column t % 64, rows 8 (t / 64) .. +7 of the tile.
-__device__ __forceinline__ void sep2d_filter(const Sep2dTile &a, const float *taps0, const float *taps1,
-                                             float *smem, float (&acc)[8]) {
+// Axis-0 factor over every staged row of a staged tile (after the staging
+// barrier) into the mid tile; ends with the barrier that publishes it.
+__device__ __forceinline__ void sep2d_rowpass(const Sep2dTile &a, const float *taps0, float *smem) {
   const int in_w = S2_TX + a.k0 - 1;
-  const int in_h = S2_TY + a.k1 - 1;
+  const int in_h = a.ty + a.k1 - 1;
   const int in_wp = in_w | 1;
   const float *s_in = smem;
-  float *s_mid = smem + sep2d_mid_offset(a.k0, a.k1);  // in_h x 64
+  float *s_mid = smem + sep2d_mid_offset(a.k0, a.k1, a.ty);  // in_h x 64
   const int tid = threadIdx.x;
   // Axis-0 factor: item = (row, quad of 4 columns).
   for (int item = tid; item < in_h * 16; item += 256) {
@@ -177,9 +182,16 @@ __device__ __forceinline__ void sep2d_filter(const Sep2dTile &a, const float *ta
     *(float4 *)(s_mid + row * S2_TX + cq) = make_float4(q[0], q[1], q[2], q[3]);
   }
   __syncthreads();
-  // Axis-1 factor: each thread owns 8 consecutive rows of one column and
-  // slides an 8-value window down the column, one LDS read per tap.
-  const int col = tid & 63, r0 = (tid >> 6) * 8;
+}
+
+// Axis-1 factor over the mid tile for the 32-row band starting at tile row h:
+// each thread owns 8 consecutive rows (h + 8 (t / 64) ..) of column t % 64
+// and slides an 8-value window down the column, one LDS read per tap.
+__device__ __forceinline__ void sep2d_colpass(const Sep2dTile &a, const float *taps1, const float *smem, int h,
+                                              float (&acc)[8]) {
+  const float *s_mid = smem + sep2d_mid_offset(a.k0, a.k1, a.ty);
+  const int tid = threadIdx.x;
+  const int col = tid & 63, r0 = h + (tid >> 6) * 8;
 #pragma unroll
   for (int o = 0; o < 8; ++o) acc[o] = 0.f;
   const float *src = s_mid + r0 * S2_TX + col;
@@ -197,13 +209,22 @@ __device__ __forceinline__ void sep2d_filter(const Sep2dTile &a, const float *ta
   }
 }
 
-// Stages and filters one tile (the caller owns the barrier before LDS is reused).
-__device__ __forceinline__ void sep2d_compute(const Sep2dTile &a, const float *taps0, const float *taps1,
-                                              float *smem, float (&acc)[8]) {
+// Both factor passes over a staged tile of height S2_TY; on return thread t
+// holds column t % 64, rows 8 (t / 64) .. +7 of the tile.
+__device__ __forceinline__ void sep2d_filter(const Sep2dTile &a, const float *taps0, const float *taps1,
+                                             float *smem, float (&acc)[8]) {
+  sep2d_rowpass(a, taps0, smem);
+  sep2d_colpass(a, taps1, smem, 0, acc);
+}
+
+// Stages, filters and stores one tile (the caller owns the barrier before
+// LDS is reused).
+__device__ __forceinline__ void sep2d_tile(const Sep2dTile &a, const float *taps0, const float *taps1,
+                                           float *smem) {
   const int in_w = S2_TX + a.k0 - 1;
-  const int in_h = S2_TY + a.k1 - 1;
+  const int in_h = a.ty + a.k1 - 1;
   const int in_wp = in_w | 1;
-  int *s_rows = (int *)(smem + sep2d_rows_offset(a.k0, a.k1));
+  int *s_rows = (int *)(smem + sep2d_rows_offset(a.k0, a.k1, a.ty));
   sep2d_rows(a, s_rows);
   __syncthreads();
   switch (a.dt_in) {
@@ -215,14 +236,12 @@ __device__ __forceinline__ void sep2d_compute(const Sep2dTile &a, const float *t
   default: sep2d_stage<IFM_U8>(a, s_rows, smem, in_w, in_h, in_wp); break;
   }
   __syncthreads();
-  sep2d_filter(a, taps0, taps1, smem, acc);
-}
-
-__device__ __forceinline__ void sep2d_tile(const Sep2dTile &a, const float *taps0, const float *taps1,
-                                           float *smem) {
-  float acc[8];
-  sep2d_compute(a, taps0, taps1, smem, acc);
-  sep2d_store(a, acc);
+  sep2d_rowpass(a, taps0, smem);
+  for (int h = 0; h < a.ty; h += S2_TY) {
+    float acc[8];
+    sep2d_colpass(a, taps1, smem, h, acc);
+    sep2d_store(a, acc, h);
+  }
 }
 
 }  // namespace ifm

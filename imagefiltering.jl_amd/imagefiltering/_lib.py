@@ -67,13 +67,21 @@ class IfmServeConfig(C.Structure):
                 ("max_inflight", C.c_int32), ("reserved", C.c_int32)]
 
 
+class IfmTileRule(C.Structure):
+    _fields_ = [("op", C.c_int32), ("dtype", C.c_int32), ("kmax", C.c_int32), ("tile_h", C.c_int32),
+                ("max_pixels", C.c_int64)]
+
+
+TILE_SEP2D = 1
+MAX_TILE_RULES = 256
+
 EXPORTS = [
     "ifm_ctx_create", "ifm_ctx_destroy", "ifm_ctx_set_stream", "ifm_ctx_synchronize", "ifm_last_error",
     "ifm_version", "ifm_padarray", "ifm_imfilter_fir", "ifm_imfilter_dense", "ifm_imfilter_fft",
     "ifm_imfilter_iir", "ifm_mapwindow_median", "ifm_mapwindow_minmax", "ifm_extrema_filter",
     "ifm_batch_run", "ifm_serve", "ifm_fill_uniform",
     "ifm_device_alloc", "ifm_device_free", "ifm_memcpy_h2d", "ifm_memcpy_d2h", "ifm_memset_d",
-    "ifm_timer_start", "ifm_timer_stop", "ifm_device_count",
+    "ifm_timer_start", "ifm_timer_stop", "ifm_device_count", "ifm_set_tile_rules", "ifm_tile_height",
 ]
 
 _lib = None
@@ -122,6 +130,8 @@ def load(path=LIBPATH):
     L.ifm_timer_start.argtypes = [vp]
     L.ifm_timer_stop.argtypes = [vp, P(C.c_float)]
     L.ifm_device_count.argtypes = [P(C.c_int)]
+    L.ifm_set_tile_rules.argtypes = [vp, P(IfmTileRule), C.c_int]
+    L.ifm_tile_height.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int64, P(C.c_int)]
     _lib = L
     return L
 
@@ -137,7 +147,13 @@ def ctx():
             if rc != OK:
                 raise RuntimeError("libifm: no usable GPU (ifm_ctx_create returned %d)" % rc)
             _ctx = h
-        return _ctx
+            new = True
+        else:
+            new = False
+    if new:
+        from . import tiles
+        tiles.load()  # tuned tile shapes of this machine, when a tuning result is cached
+    return _ctx
 
 
 def check(rc):

@@ -167,3 +167,28 @@ def stats(mix, done):
             "gpix_per_s": float(request_pixels(mix).sum()) / t_end / 1e9 if t_end > 0 else None,
             "hbm_gbs": float(request_bytes(mix).sum()) / t_end / 1e9 if t_end > 0 else None,
             "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3)}
+
+
+def autotune(mix, budget_bytes=64 << 30, sample=3000, batches=(256, 512, 1024, 2048), inflight=(1, 2, 4, 7),
+             seed=20261208):
+    """Schedule autotuner for the request loop: runs closed-loop trials of a
+    seeded sample of the stream (requests drawn from `mix`, all arriving at
+    once) for every (max_batch, max_inflight) pair and returns the pair with the
+    highest images/s, with the trial table.  Each trial reuses the same sample,
+    so the pairs are compared on identical work.  max_inflight is capped at 7
+    by the request loop (eight descriptor slots, one kept free for the batch
+    being built)."""
+    n = mix["n"]
+    rng = np.random.default_rng(seed)
+    idx = np.sort(rng.choice(n, size=min(sample, n), replace=False))
+    sub = {k: (v[idx] if isinstance(v, np.ndarray) else v) for k, v in mix.items()}
+    sub["n"] = len(idx)
+    sub["arrival"] = np.zeros(len(idx))
+    serve(sub, "batched", budget_bytes=budget_bytes)  # warm-up (allocations, code objects)
+    table = []
+    for mb in batches:
+        for mi in inflight:
+            _, done = serve(sub, "batched", budget_bytes=budget_bytes, max_batch=mb, max_inflight=mi)
+            table.append({"max_batch": mb, "max_inflight": mi, "images_per_s": stats(sub, done)["images_per_s"]})
+    best = max(table, key=lambda r: r["images_per_s"] or 0.0)
+    return {"max_batch": best["max_batch"], "max_inflight": best["max_inflight"]}, table

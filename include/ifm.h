@@ -236,6 +236,29 @@ int ifm_serve(ifm_ctx *ctx, const ifm_request *reqs, const double *arrival_s, in
  * counter hash times 2^-24 (exact in FP32; BF16 / FP16 round to nearest even). */
 int ifm_fill_uniform(ifm_ctx *ctx, ifm_array *a, uint64_t seed);
 
+/* ---- tile shapes -------------------------------------------------------------
+ * The fused two-factor separable kernel (single-image imfilter over axes 0 and 1,
+ * and the separable op of ifm_batch_run / ifm_serve) computes 64 x tile_h output
+ * tiles, tile_h = 32 or 64.  The host's autotuner times both per (input dtype,
+ * longest factor, image size) bucket and installs the winners as rules; the
+ * first rule that applies to a call picks its tile_h, and calls no rule covers
+ * use 32.  Results do not depend on the tile shape (same taps, same order).
+ * No reference counterpart: ImageFiltering's FIRTiled picks tile sizes from
+ * TiledIteration's cache heuristic (src/imfilter.jl:1298-1312). */
+typedef enum { IFM_TILE_SEP2D = 1 } ifm_tile_op;
+typedef struct {
+  int32_t op;          /* ifm_tile_op */
+  int32_t dtype;       /* input ifm_dtype, or -1 for any */
+  int32_t kmax;        /* applies when the longer of the two factors has <= kmax taps */
+  int32_t tile_h;      /* 32 or 64 */
+  int64_t max_pixels;  /* applies when the output plane has <= max_pixels pixels */
+} ifm_tile_rule;
+#define IFM_MAX_TILE_RULES 256
+/* Replaces the ctx's rules (n = 0 clears them). */
+int ifm_set_tile_rules(ifm_ctx *ctx, const ifm_tile_rule *rules, int n);
+/* The tile height a call with these parameters would use. */
+int ifm_tile_height(ifm_ctx *ctx, int op, int dtype, int kmax, int64_t pixels, int *tile_h);
+
 /* ---- device memory plumbing ------------------------------------------------ */
 /* Helpers for hosts without their own HIP allocator (the Python package and
  * the Julia binding use them for device-resident arrays).  Copies are

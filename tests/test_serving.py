@@ -185,3 +185,16 @@ def test_batch_rejects_bad_requests():
     bad[0].window = 9
     with pytest.raises(NotImplementedError):
         _lib.check(L.ifm_batch_run(ctx, bad, 1))
+
+
+@pytest.mark.gpu
+def test_schedule_autotuner_picks_a_tried_schedule():
+    mix = serving.request_mix(80, seed=11, side=(32, 400))
+    sched, table = serving.autotune(mix, budget_bytes=1 << 30, sample=40, batches=(8, 32), inflight=(1, 3))
+    assert len(table) == 4
+    assert {(r["max_batch"], r["max_inflight"]) for r in table} == {(8, 1), (8, 3), (32, 1), (32, 3)}
+    best = max(r["images_per_s"] for r in table)
+    assert [r for r in table if (r["max_batch"], r["max_inflight"]) == (sched["max_batch"], sched["max_inflight"])][0][
+        "images_per_s"] == best
+    admit, done = serving.serve(mix, "batched", budget_bytes=1 << 30, **sched)
+    assert np.all(done >= admit)

@@ -3,9 +3,12 @@
 // Follows _imfilter_dim! (src/imfilter.jl:964-1024) with leftborder!
 // (:1027-1048) and rightborder! (:1051-1092): forward recursion with the
 // steady-state left initialisation, right-edge initialisation by the 3x3
-// matrix M (Triggs & Sdika Eqs. 14-15), backward recursion and scaling.  A
-// line is sequential, so each thread owns one line; lines along non-
-// contiguous axes are adjacent in memory across the wave (coalesced).
+// matrix M (Triggs & Sdika Eqs. 14-15), backward recursion and scaling.
+// Lines along the contiguous axis run k_iir_rows (LDS-staged chunks); lines
+// along other axes run k_iir_seg (whole lines in registers, the axis split
+// across waves) up to 512 elements, k_iir_strided (one thread per line)
+// beyond.  Lanes sit on adjacent lines, which are adjacent in memory for the
+// strided axes (coalesced).
 #include "ifm_internal.h"
 
 namespace ifm {
@@ -251,6 +254,172 @@ __global__ void __launch_bounds__(256, 2) k_iir_rows(IirArgs p) {
   }
 }
 
+// Lines along a strided axis, held in registers.  A workgroup owns 64
+// adjacent lines (lane l: line 64 blockIdx.x + l; adjacent lines are adjacent
+// in memory, so a wave's access to one position is one coalesced row) and
+// cuts the axis into nw segments of at most IS_S elements, one wave each, so
+// every element is read once and written once (the one-thread-per-line form
+// writes the forward pass out and reads it back).  The recursions are linear:
+// a segment's values are its recursion from zero state plus the homogeneous
+// response to the state entering it.  Each wave runs its segment from zero
+// state; the waves exchange their end states through LDS; each composes the
+// state entering its segment from those with the 3x3 transfer matrix of a
+// full segment, and adds sum_c h_c[j] s_c.  The first segment's forward pass
+// and the last segment's backward pass start from the reference's border
+// initialisation directly (the um / M start of k_iir_strided).
+constexpr int IS_S = 64;     // most elements per segment (doubles in registers per lane)
+constexpr int IS_MAXW = 8;   // most segments (waves): n <= 512
+
+struct IirSegTables {
+  double h[3][IS_S];  // forward homogeneous responses: state (y[-1], y[-2], y[-3]) = e_c
+  double g[3][IS_S];  // backward, by distance from the segment's right end
+  int seg;            // segment length (the last may be shorter, >= 3)
+  int nw;             // segments
+};
+
+// LDS (doubles): tables h, g (6 IS_S), then forward and backward end states
+// [wave][component][lane] (2 x nw x 3 x 64).
+inline size_t iir_seg_lds_doubles(int nw) { return 6 * IS_S + 2 * (size_t)nw * 3 * 64; }
+
+template <int DT>
+__global__ void __launch_bounds__(64 * IS_MAXW) k_iir_seg(IirArgs p, IirSegTables tb) {
+  extern __shared__ double sm[];
+  double *hs = sm, *gs = sm + 3 * IS_S;
+  double *stf = sm + 6 * IS_S, *stb = stf + tb.nw * 3 * 64;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nw = tb.nw, S = tb.seg;
+  const int n = p.n[p.axis];
+  const int s0 = w * S;
+  const int L = min(S, n - s0);
+  const bool first = w == 0, last = w == nw - 1;
+  const int64_t line = (int64_t)blockIdx.x * 64 + lane;
+  const bool ok = line < p.lines;
+  const int64_t st = p.stride;
+  const int64_t base = (line % st) + (line / st) * st * n + (int64_t)s0 * st;
+  double v[IS_S];
+#pragma unroll
+  for (int j = 0; j < IS_S; ++j)
+    if (j < L && ok) v[j] = iir_ld<DT>(p, base + (int64_t)j * st);
+  for (int i = threadIdx.x; i < 6 * IS_S; i += blockDim.x) sm[i] = i < 3 * IS_S ? tb.h[i / IS_S][i % IS_S]
+                                                                                   : tb.g[i / IS_S - 3][i % IS_S];
+  const double a1 = p.a[0], a2 = p.a[1], a3 = p.a[2];
+  const double b1 = p.b[0], b2 = p.b[1], b3 = p.b[2];
+  const double asum = a1 + a2 + a3, bsum = b1 + b2 + b3;
+  // forward: the first segment from y[-k] = u- (leftborder!), the others from zero
+  double ym1 = 0.0, ym2 = 0.0, ym3 = 0.0, xlast = 0.0;
+  if (first) {
+    const double iminus = p.isfill ? p.fill : v[0];
+    ym1 = ym2 = ym3 = iminus / (1 - asum);
+  }
+#pragma unroll
+  for (int j = 0; j < IS_S; ++j) {
+    if (j < L) {
+      if (j == L - 1) xlast = v[j];
+      const double y = v[j] + a1 * ym1 + a2 * ym2 + a3 * ym3;
+      v[j] = y;
+      ym3 = ym2;
+      ym2 = ym1;
+      ym1 = y;
+    }
+  }
+  stf[(w * 3 + 0) * 64 + lane] = ym1;
+  stf[(w * 3 + 1) * 64 + lane] = ym2;
+  stf[(w * 3 + 2) * 64 + lane] = ym3;
+  __syncthreads();
+  // state entering segment w: T_0 = end of segment 0, T_u = Phi T_{u-1} + P_u
+  // with Phi[r][c] = h_c[S - 1 - r]
+  double e0 = ym1, e1 = ym2, e2 = ym3;  // true end state of this segment
+  if (!first) {
+    double t0 = stf[lane], t1 = stf[64 + lane], t2 = stf[128 + lane];
+    for (int u = 1; u < w; ++u) {
+      const double *q = stf + u * 192;
+      const double n0 = hs[S - 1] * t0 + hs[IS_S + S - 1] * t1 + hs[2 * IS_S + S - 1] * t2 + q[lane];
+      const double n1 = hs[S - 2] * t0 + hs[IS_S + S - 2] * t1 + hs[2 * IS_S + S - 2] * t2 + q[64 + lane];
+      const double n2 = hs[S - 3] * t0 + hs[IS_S + S - 3] * t1 + hs[2 * IS_S + S - 3] * t2 + q[128 + lane];
+      t0 = n0;
+      t1 = n1;
+      t2 = n2;
+    }
+#pragma unroll
+    for (int j = 0; j < IS_S; ++j)
+      if (j < L) v[j] += hs[j] * t0 + hs[IS_S + j] * t1 + hs[2 * IS_S + j] * t2;
+    e0 = ym1 + hs[L - 1] * t0 + hs[IS_S + L - 1] * t1 + hs[2 * IS_S + L - 1] * t2;
+    e1 = ym2 + hs[L - 2] * t0 + hs[IS_S + L - 2] * t1 + hs[2 * IS_S + L - 2] * t2;
+    e2 = ym3 + hs[L - 3] * t0 + hs[IS_S + L - 3] * t1 + hs[2 * IS_S + L - 3] * t2;
+  }
+  // backward: the last segment from the rightborder! values (e = y[n-1],
+  // y[n-2], y[n-3]), the others from zero
+  double z0 = 0.0, z1 = 0.0, z2 = 0.0;
+  if (last) {
+    const double iplus = p.isfill ? p.fill : xlast;
+    const double uplus = iplus / (1 - asum);
+    const double vplus = uplus / (1 - bsum);
+    const double d0 = e0 - uplus, d1 = e1 - uplus, d2 = e2 - uplus;
+    const double m0 = p.M[0] * d0 + p.M[1] * d1 + p.M[2] * d2 + vplus;
+    const double m1 = p.M[3] * d0 + p.M[4] * d1 + p.M[5] * d2 + vplus;
+    const double m2 = p.M[6] * d0 + p.M[7] * d1 + p.M[8] * d2 + vplus;
+    z2 = m0;
+    z1 = e1 + b1 * z2 + b2 * m1 + b3 * m2;
+    z0 = e2 + b1 * z1 + b2 * z2 + b3 * m1;
+  }
+  double zp1 = 0.0, zp2 = 0.0, zp3 = 0.0;
+#pragma unroll
+  for (int j = IS_S - 1; j >= 0; --j) {
+    if (j < L) {
+      double z;
+      if (last && j == L - 1) z = z2;
+      else if (last && j == L - 2) z = z1;
+      else if (last && j == L - 3) z = z0;
+      else z = v[j] + b1 * zp1 + b2 * zp2 + b3 * zp3;
+      v[j] = z;
+      zp3 = zp2;
+      zp2 = zp1;
+      zp1 = z;
+    }
+  }
+  stb[(w * 3 + 0) * 64 + lane] = zp1;
+  stb[(w * 3 + 1) * 64 + lane] = zp2;
+  stb[(w * 3 + 2) * 64 + lane] = zp3;
+  __syncthreads();
+  // state entering segment w from the right: R_{nw-1} = left end of the last
+  // segment, R_u = Psi R_{u+1} + Q_u with Psi[r][c] = g_c[S - 1 - r]
+  if (!last) {
+    const double *q0 = stb + (nw - 1) * 192;
+    double t0 = q0[lane], t1 = q0[64 + lane], t2 = q0[128 + lane];
+    for (int u = nw - 2; u > w; --u) {
+      const double *q = stb + u * 192;
+      const double n0 = gs[S - 1] * t0 + gs[IS_S + S - 1] * t1 + gs[2 * IS_S + S - 1] * t2 + q[lane];
+      const double n1 = gs[S - 2] * t0 + gs[IS_S + S - 2] * t1 + gs[2 * IS_S + S - 2] * t2 + q[64 + lane];
+      const double n2 = gs[S - 3] * t0 + gs[IS_S + S - 3] * t1 + gs[2 * IS_S + S - 3] * t2 + q[128 + lane];
+      t0 = n0;
+      t1 = n1;
+      t2 = n2;
+    }
+#pragma unroll
+    for (int j = 0; j < IS_S; ++j)
+      if (j < L) v[j] += gs[L - 1 - j] * t0 + gs[IS_S + L - 1 - j] * t1 + gs[2 * IS_S + L - 1 - j] * t2;
+  }
+  const double sc = p.scale;
+  if (ok) {
+#pragma unroll
+    for (int j = 0; j < IS_S; ++j)
+      if (j < L) p.dst[base + (int64_t)j * st] = v[j] * sc;
+  }
+}
+
+// Segmentation of an axis of n elements: nw waves of seg elements (the last
+// one shorter, at least 3); false if n is too long for one workgroup.
+static bool iir_segments(int n, int *seg, int *nw) {
+  const int w = (n + IS_S - 1) / IS_S;
+  if (w > IS_MAXW || n < 4) return false;
+  const int s = (n + w - 1) / w;
+  if (n - (w - 1) * s < 3) return false;
+  *seg = s;
+  *nw = w;
+  return true;
+}
+
 hipError_t launch_iir(const Dense4 &in, Dense4 &out, int axis, const IirCoef &c, int isfill, double fill,
                       hipStream_t st) {
   IirArgs p;
@@ -280,6 +449,36 @@ hipError_t launch_iir(const Dense4 &in, Dense4 &out, int axis, const IirCoef &c,
     if (in.dtype == IFM_F32) hipLaunchKernelGGL(k_iir_rows<IFM_F32>, dim3(nb), dim3(256), lds, st, p);
     else if (in.dtype == IFM_F64) hipLaunchKernelGGL(k_iir_rows<IFM_F64>, dim3(nb), dim3(256), lds, st, p);
     else hipLaunchKernelGGL(k_iir_rows<-1>, dim3(nb), dim3(256), lds, st, p);
+    return hipGetLastError();
+  }
+  int seg, nw;
+  if (iir_segments(p.n[axis], &seg, &nw)) {  // strided lines held in registers
+    IirSegTables tb;
+    tb.seg = seg;
+    tb.nw = nw;
+    for (int k = 0; k < 3; ++k) {
+      // responses to the unit state e_k: y[-1 - k] = 1
+      double hm[3] = {0.0, 0.0, 0.0}, gm[3] = {0.0, 0.0, 0.0};
+      hm[k] = gm[k] = 1.0;
+      for (int j = 0; j < IS_S; ++j) {
+        const double h = c.a[0] * hm[0] + c.a[1] * hm[1] + c.a[2] * hm[2];
+        const double g = c.b[0] * gm[0] + c.b[1] * gm[1] + c.b[2] * gm[2];
+        tb.h[k][j] = h;
+        tb.g[k][j] = g;
+        hm[2] = hm[1];
+        hm[1] = hm[0];
+        hm[0] = h;
+        gm[2] = gm[1];
+        gm[1] = gm[0];
+        gm[0] = g;
+      }
+    }
+    const unsigned nb = (unsigned)((p.lines + 63) / 64);
+    const size_t lds = sizeof(double) * iir_seg_lds_doubles(nw);
+    const dim3 blk(64 * nw);
+    if (in.dtype == IFM_F32) hipLaunchKernelGGL(k_iir_seg<IFM_F32>, dim3(nb), blk, lds, st, p, tb);
+    else if (in.dtype == IFM_F64) hipLaunchKernelGGL(k_iir_seg<IFM_F64>, dim3(nb), blk, lds, st, p, tb);
+    else hipLaunchKernelGGL(k_iir_seg<-1>, dim3(nb), blk, lds, st, p, tb);
     return hipGetLastError();
   }
   int64_t blocks = (p.lines + 255) / 256;

@@ -480,6 +480,25 @@ def test_iir_contiguous_lines_chunk_edges():
             np.testing.assert_allclose(out, want, rtol=1e-9, atol=1e-12, err_msg="%s %s" % (shape, border))
 
 
+def test_iir_strided_lines_segments():
+    # strided axes cut into per-wave segments: one segment (n <= 64), two or
+    # more (65, 129, 200, 512; last segment shorter), the longest segmented
+    # axis (512) and the one-thread-per-line form beyond it (513), partial
+    # groups of 64 lines, long and short sigma, F32 / F64 input
+    rng = np.random.default_rng(29)
+    # (every axis is longer than 3: the reference rejects shorter ones)
+    for shape, dt, sg in [((4, 65), np.float32, 2.0), ((70, 129), np.float64, 6.0), ((5, 64, 4), np.float32, 1.0),
+                          ((4, 513), np.float32, 3.0), ((9, 200, 7), np.float64, 12.0), ((66, 512), np.float32, 2.0),
+                          ((4, 4, 130), np.float32, 1.5)]:
+        x = np.asfortranarray(rng.random(shape).astype(dt))
+        kern = KernelFactors.IIRGaussian(tuple(sg for _ in shape))
+        for border in ["replicate", IF.Fill(0.25)]:
+            out = IF.imfilter(x, kern, border)
+            ob = ("fill", border.value) if isinstance(border, IF.Fill) else "replicate"
+            want = O.imfilter_iir(x.astype(np.float64), iir_spec(kern), ob)
+            np.testing.assert_allclose(out, want, rtol=1e-9, atol=1e-12, err_msg="%s %s" % (shape, border))
+
+
 def test_iir_per_dim_call_and_errors():
     imgf = np.zeros((5, 7))
     imgf[2, 3] = 1

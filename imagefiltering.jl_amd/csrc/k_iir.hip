@@ -278,10 +278,20 @@ struct IirSegTables {
 };
 
 // LDS (doubles): tables h, g (6 IS_S), then forward and backward end states
-// [wave][component][lane] (2 x nw x 3 x 64).
-inline size_t iir_seg_lds_doubles(int nw) { return 6 * IS_S + 2 * (size_t)nw * 3 * 64; }
+// [wave][component][lane] (2 x nw x 3 x 64), then (lines along the contiguous
+// axis) one transpose buffer per wave of IS_TR lines x IS_TP.
+constexpr int IS_TR = 16;          // lines per transpose round (lanes 16 r .. 16 r + 15 in round r)
+constexpr int IS_TP = IS_S + 1;    // transpose buffer pitch (doubles): conflict-free both ways
+__host__ __device__ inline size_t iir_seg_lds_doubles(int nw, bool rows) {
+  return 6 * IS_S + 2 * (size_t)nw * 3 * 64 + (rows ? (size_t)nw * IS_TR * IS_TP : 0);
+}
 
-template <int DT>
+// ROWS: lines along the contiguous axis.  A line's segment is then IS_S
+// consecutive elements, so a wave moves it through its transpose buffer in
+// rounds of IS_TR lines: lanes run along the segment for the global loads
+// and stores (one contiguous run of the line per instruction), and along the
+// lines for the recursion.
+template <int DT, bool ROWS>
 __global__ void __launch_bounds__(64 * IS_MAXW) k_iir_seg(IirArgs p, IirSegTables tb) {
   extern __shared__ double sm[];
   double *hs = sm, *gs = sm + 3 * IS_S;
@@ -298,9 +308,31 @@ __global__ void __launch_bounds__(64 * IS_MAXW) k_iir_seg(IirArgs p, IirSegTable
   const int64_t st = p.stride;
   const int64_t base = (line % st) + (line / st) * st * n + (int64_t)s0 * st;
   double v[IS_S];
+  double *tr = sm + iir_seg_lds_doubles(nw, false) + w * IS_TR * IS_TP;  // ROWS only
+  if constexpr (ROWS) {
+    const int64_t line0 = (int64_t)blockIdx.x * 64;
+    for (int r = 0; r < 64 / IS_TR; ++r) {
+      double x[IS_TR];
 #pragma unroll
-  for (int j = 0; j < IS_S; ++j)
-    if (j < L && ok) v[j] = iir_ld<DT>(p, base + (int64_t)j * st);
+      for (int i = 0; i < IS_TR; ++i) {
+        const int64_t ln = line0 + r * IS_TR + i;
+        x[i] = (lane < L && ln < p.lines) ? iir_ld<DT>(p, ln * n + s0 + lane) : 0.0;
+      }
+#pragma unroll
+      for (int i = 0; i < IS_TR; ++i) tr[i * IS_TP + lane] = x[i];
+      __syncthreads();
+      if ((lane >> 4) == r) {
+#pragma unroll
+        for (int j = 0; j < IS_S; ++j)
+          if (j < L) v[j] = tr[(lane & (IS_TR - 1)) * IS_TP + j];
+      }
+      __syncthreads();
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < IS_S; ++j)
+      if (j < L && ok) v[j] = iir_ld<DT>(p, base + (int64_t)j * st);
+  }
   for (int i = threadIdx.x; i < 6 * IS_S; i += blockDim.x) sm[i] = i < 3 * IS_S ? tb.h[i / IS_S][i % IS_S]
                                                                                    : tb.g[i / IS_S - 3][i % IS_S];
   const double a1 = p.a[0], a2 = p.a[1], a3 = p.a[2];
@@ -401,7 +433,23 @@ __global__ void __launch_bounds__(64 * IS_MAXW) k_iir_seg(IirArgs p, IirSegTable
       if (j < L) v[j] += gs[L - 1 - j] * t0 + gs[IS_S + L - 1 - j] * t1 + gs[2 * IS_S + L - 1 - j] * t2;
   }
   const double sc = p.scale;
-  if (ok) {
+  if constexpr (ROWS) {
+    const int64_t line0 = (int64_t)blockIdx.x * 64;
+    for (int r = 0; r < 64 / IS_TR; ++r) {
+      if ((lane >> 4) == r) {
+#pragma unroll
+        for (int j = 0; j < IS_S; ++j)
+          if (j < L) tr[(lane & (IS_TR - 1)) * IS_TP + j] = v[j] * sc;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < IS_TR; ++i) {
+        const int64_t ln = line0 + r * IS_TR + i;
+        if (lane < L && ln < p.lines) p.dst[ln * n + s0 + lane] = tr[i * IS_TP + lane];
+      }
+      __syncthreads();
+    }
+  } else if (ok) {
 #pragma unroll
     for (int j = 0; j < IS_S; ++j)
       if (j < L) p.dst[base + (int64_t)j * st] = v[j] * sc;
@@ -443,16 +491,11 @@ hipError_t launch_iir(const Dense4 &in, Dense4 &out, int axis, const IirCoef &c,
   p.isfill = isfill;
   p.fill = fill;
   if (p.lines == 0) return hipSuccess;
-  if (stride == 1) {  // lines along the contiguous axis: LDS-staged chunks
-    const unsigned nb = (unsigned)((p.lines + IR_L - 1) / IR_L);
-    const size_t lds = sizeof(double) * IR_L * IR_P;
-    if (in.dtype == IFM_F32) hipLaunchKernelGGL(k_iir_rows<IFM_F32>, dim3(nb), dim3(256), lds, st, p);
-    else if (in.dtype == IFM_F64) hipLaunchKernelGGL(k_iir_rows<IFM_F64>, dim3(nb), dim3(256), lds, st, p);
-    else hipLaunchKernelGGL(k_iir_rows<-1>, dim3(nb), dim3(256), lds, st, p);
-    return hipGetLastError();
-  }
   int seg, nw;
-  if (iir_segments(p.n[axis], &seg, &nw)) {  // strided lines held in registers
+  const bool typed = in.dtype == IFM_F32 || in.dtype == IFM_F64;
+  // whole lines held in registers (contiguous lines: F32 / F64 input only; the
+  // run-time element type's loads do not fit the transpose rounds' registers)
+  if (iir_segments(p.n[axis], &seg, &nw) && (stride != 1 || typed)) {
     IirSegTables tb;
     tb.seg = seg;
     tb.nw = nw;
@@ -473,12 +516,26 @@ hipError_t launch_iir(const Dense4 &in, Dense4 &out, int axis, const IirCoef &c,
         gm[0] = g;
       }
     }
+    const bool rows = stride == 1;
     const unsigned nb = (unsigned)((p.lines + 63) / 64);
-    const size_t lds = sizeof(double) * iir_seg_lds_doubles(nw);
+    const size_t lds = sizeof(double) * iir_seg_lds_doubles(nw, rows);
     const dim3 blk(64 * nw);
-    if (in.dtype == IFM_F32) hipLaunchKernelGGL(k_iir_seg<IFM_F32>, dim3(nb), blk, lds, st, p, tb);
-    else if (in.dtype == IFM_F64) hipLaunchKernelGGL(k_iir_seg<IFM_F64>, dim3(nb), blk, lds, st, p, tb);
-    else hipLaunchKernelGGL(k_iir_seg<-1>, dim3(nb), blk, lds, st, p, tb);
+    if (rows) {
+      if (in.dtype == IFM_F32) hipLaunchKernelGGL((k_iir_seg<IFM_F32, true>), dim3(nb), blk, lds, st, p, tb);
+      else hipLaunchKernelGGL((k_iir_seg<IFM_F64, true>), dim3(nb), blk, lds, st, p, tb);
+    } else {
+      if (in.dtype == IFM_F32) hipLaunchKernelGGL((k_iir_seg<IFM_F32, false>), dim3(nb), blk, lds, st, p, tb);
+      else if (in.dtype == IFM_F64) hipLaunchKernelGGL((k_iir_seg<IFM_F64, false>), dim3(nb), blk, lds, st, p, tb);
+      else hipLaunchKernelGGL((k_iir_seg<-1, false>), dim3(nb), blk, lds, st, p, tb);
+    }
+    return hipGetLastError();
+  }
+  if (stride == 1) {  // lines along the contiguous axis: LDS-staged chunks
+    const unsigned nb = (unsigned)((p.lines + IR_L - 1) / IR_L);
+    const size_t lds = sizeof(double) * IR_L * IR_P;
+    if (in.dtype == IFM_F32) hipLaunchKernelGGL(k_iir_rows<IFM_F32>, dim3(nb), dim3(256), lds, st, p);
+    else if (in.dtype == IFM_F64) hipLaunchKernelGGL(k_iir_rows<IFM_F64>, dim3(nb), dim3(256), lds, st, p);
+    else hipLaunchKernelGGL(k_iir_rows<-1>, dim3(nb), dim3(256), lds, st, p);
     return hipGetLastError();
   }
   int64_t blocks = (p.lines + 255) / 256;

@@ -481,15 +481,15 @@ def test_iir_contiguous_lines_chunk_edges():
 
 
 def test_iir_strided_lines_segments():
-    # strided axes cut into per-wave segments: one segment (n <= 64), two or
-    # more (65, 129, 200, 512; last segment shorter), the longest segmented
-    # axis (512) and the one-thread-per-line form beyond it (513), partial
+    # axes cut into per-wave segments: one segment (n <= 64), two or more
+    # (65, 129, 200, 512; last segment shorter), the longest segmented axis
+    # (512) and the forms beyond it (513 strided, 600 contiguous), partial
     # groups of 64 lines, long and short sigma, F32 / F64 input
     rng = np.random.default_rng(29)
     # (every axis is longer than 3: the reference rejects shorter ones)
     for shape, dt, sg in [((4, 65), np.float32, 2.0), ((70, 129), np.float64, 6.0), ((5, 64, 4), np.float32, 1.0),
                           ((4, 513), np.float32, 3.0), ((9, 200, 7), np.float64, 12.0), ((66, 512), np.float32, 2.0),
-                          ((4, 4, 130), np.float32, 1.5)]:
+                          ((4, 4, 130), np.float32, 1.5), ((600, 5), np.float32, 2.0), ((512, 70), np.float64, 2.5)]:
         x = np.asfortranarray(rng.random(shape).astype(dt))
         kern = KernelFactors.IIRGaussian(tuple(sg for _ in shape))
         for border in ["replicate", IF.Fill(0.25)]:

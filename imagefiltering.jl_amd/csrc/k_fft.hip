@@ -49,9 +49,25 @@ struct FftLinesArgs {
   int radix[FFT_MAXPASS];
 };
 
+// j / Ns and j % Ns for j < 2^13 by a float reciprocal, corrected by one step
+// (the product's error is far below one unit at these sizes).
+__device__ __forceinline__ void divmod_small(int j, int ns, float rns, int &q, int &r) {
+  q = (int)((float)j * rns);
+  r = j - q * ns;
+  if (r < 0) {
+    --q;
+    r += ns;
+  } else if (r >= ns) {
+    ++q;
+    r -= ns;
+  }
+}
+
 template <int R>
 __device__ __forceinline__ void fft_pass(float2 *s, const float2 *tw, int N, int Ns, int sign) {
   const int NR = N / R;
+  const float rns = 1.0f / (float)Ns;
+  const int unit = N / (Ns * R);
   constexpr int J = (R == 2) ? FFT_MAXJ : (R == 3 ? (FFT_MAXN / 3 + FFT_THREADS - 1) / FFT_THREADS
                                                   : FFT_MAXN / (4 * FFT_THREADS));
   float2 v[J][R];
@@ -60,13 +76,14 @@ __device__ __forceinline__ void fft_pass(float2 *s, const float2 *tw, int N, int
   for (int c = 0; c < J; ++c) {
     const int j = tid + c * FFT_THREADS;
     if (j < NR) {
-      const int k = j % Ns;
-      const int step = (N / (Ns * R)) * k;  // twiddle exponent unit
+      int q, k;
+      divmod_small(j, Ns, rns, q, k);
+      const int step = unit * k;  // twiddle exponent unit: step * i < N
 #pragma unroll
       for (int i = 0; i < R; ++i) {
         float2 x = s[j + i * NR];
         if (i > 0) {
-          float2 w = tw[(step * i) % N];
+          float2 w = tw[step * i];
           if (sign > 0) w.y = -w.y;
           x = cmulf(x, w);
         }
@@ -105,8 +122,9 @@ __device__ __forceinline__ void fft_pass(float2 *s, const float2 *tw, int N, int
   for (int c = 0; c < J; ++c) {
     const int j = tid + c * FFT_THREADS;
     if (j < NR) {
-      const int k = j % Ns;
-      const int base = (j / Ns) * Ns * R + k;
+      int q, k;
+      divmod_small(j, Ns, rns, q, k);
+      const int base = q * Ns * R + k;
 #pragma unroll
       for (int i = 0; i < R; ++i) s[base + i * Ns] = v[c][i];
     }

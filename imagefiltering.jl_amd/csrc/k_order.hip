@@ -49,6 +49,63 @@ struct NetGeom {
   static constexpr int VW = TX + K - 1, VH = TY + K - 1;
 };
 
+// Order key of an element of type DT (key32 with the type fixed).
+template <int DT>
+__device__ __forceinline__ uint32_t key_of(const void *p, int64_t i) {
+  if constexpr (DT == IFM_U8) {
+    return ((const uint8_t *)p)[i];
+  } else if constexpr (DT == IFM_F32) {
+    const uint32_t u = __float_as_uint(((const float *)p)[i]);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return 0xffffffffu;
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+  } else {
+    const uint32_t u = ((const uint16_t *)p)[i];
+    constexpr uint32_t expmask = DT == IFM_BF16 ? 0x7f80u : 0x7c00u;
+    if ((u & 0x7fffu) > expmask) return 0xffffu;
+    return (u & 0x8000u) ? (~u & 0xffffu) : (u | 0x8000u);
+  }
+}
+
+// Stages the LH x LW key tile whose element (0, 0) is image position
+// (gx, gy) before the border map.  Lane c of a wave owns tile columns c and
+// c + 64 (their border maps computed once), wave w rows w, w + 4, ...: a row's
+// map is wave-uniform, one load instruction reads a contiguous row piece, and
+// every load of the thread is issued before the first is used.
+template <int DT, int LW, int LH>
+__device__ __forceinline__ void median_stage(const MedianArgs &a, int gx, int gy, int64_t ibase, uint32_t *tile) {
+  static_assert(LW <= 128, "two columns per lane");
+  constexpr int NR = (LH + 3) / 4;
+  const int c = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  int sx[2];
+  bool cok[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    cok[h] = c + 64 * h < LW;
+    sx[h] = cok[h] ? bmap(a.style, gx + c + 64 * h, a.n0) : -1;
+  }
+  uint32_t k[NR][2];
+#pragma unroll
+  for (int j = 0; j < NR; ++j) {
+    const int r = w + 4 * j;
+    const int sy = r < LH ? bmap(a.style, gy + r, a.n1) : -1;  // wave-uniform
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      k[j][h] = a.fillkey;
+      if (sy >= 0 && sx[h] >= 0) k[j][h] = key_of<DT>(a.in, ibase + (int64_t)sy * a.n0 + sx[h]);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NR; ++j) {
+    const int r = w + 4 * j;
+    if (r < LH) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        if (cok[h]) tile[r * LW + c + 64 * h] = k[j][h];
+    }
+  }
+}
+
 // One BX x BY output tile whose first output is (bx, by) in output indices,
 // image coordinate (a.f0 + bx, a.f1 + by), of the plane at ibase / obase.
 template <int K, int TX, int TY, int PACK>
@@ -59,11 +116,12 @@ __device__ __forceinline__ void median_tile(const MedianArgs &a, int bx, int by,
   const int tid = threadIdx.x;
   const int tx = tid & 15, ty = tid >> 4;
   const int x0 = a.f0 + bx, y0 = a.f1 + by;
-  for (int i = tid; i < LH * LW; i += 256) {
-    int r = i / LW, c = i - r * LW;
-    int sy = bmap(a.style, y0 - G::H + r, a.n1);
-    int sx = bmap(a.style, x0 - G::H + c, a.n0);
-    tile[i] = (sy < 0 || sx < 0) ? a.fillkey : key32(a.in, ibase + (int64_t)sy * a.n0 + sx, a.dt);
+  switch (a.dt) {
+  case IFM_U8:
+  case IFM_N0F8: median_stage<IFM_U8, LW, LH>(a, x0 - G::H, y0 - G::H, ibase, tile); break;
+  case IFM_F32: median_stage<IFM_F32, LW, LH>(a, x0 - G::H, y0 - G::H, ibase, tile); break;
+  case IFM_BF16: median_stage<IFM_BF16, LW, LH>(a, x0 - G::H, y0 - G::H, ibase, tile); break;
+  default: median_stage<IFM_F16, LW, LH>(a, x0 - G::H, y0 - G::H, ibase, tile); break;
   }
   __syncthreads();
   const int cx = tx * TX, cy = ty * TY;

@@ -70,8 +70,10 @@ __device__ __forceinline__ uint32_t key_of(const void *p, int64_t i) {
 // (gx, gy) before the border map.  Lane c of a wave owns tile columns c and
 // c + 64 (their border maps computed once), wave w rows w, w + 4, ...: a row's
 // map is wave-uniform, one load instruction reads a contiguous row piece, and
-// every load of the thread is issued before the first is used.
-template <int DT, int LW, int LH>
+// every load of the thread is issued before the first is used.  BYH > 0
+// (16-bit keys, two tiles per register): word (p, c) holds the keys of rows p
+// and p + BYH in its low and high halves, the layout the packed network reads.
+template <int DT, int LW, int LH, int BYH>
 __device__ __forceinline__ void median_stage(const MedianArgs &a, int gx, int gy, int64_t ibase, uint32_t *tile) {
   static_assert(LW <= 128, "two columns per lane");
   constexpr int NR = (LH + 3) / 4;
@@ -100,8 +102,16 @@ __device__ __forceinline__ void median_stage(const MedianArgs &a, int gx, int gy
     const int r = w + 4 * j;
     if (r < LH) {
 #pragma unroll
-      for (int h = 0; h < 2; ++h)
-        if (cok[h]) tile[r * LW + c + 64 * h] = k[j][h];
+      for (int h = 0; h < 2; ++h) {
+        if (!cok[h]) continue;
+        if constexpr (BYH > 0) {
+          uint16_t *t16 = (uint16_t *)tile;
+          if (r < LH - BYH) t16[2 * (r * LW + c + 64 * h)] = (uint16_t)k[j][h];
+          if (r >= BYH) t16[2 * ((r - BYH) * LW + c + 64 * h) + 1] = (uint16_t)k[j][h];
+        } else {
+          tile[r * LW + c + 64 * h] = k[j][h];
+        }
+      }
     }
   }
 }
@@ -116,12 +126,13 @@ __device__ __forceinline__ void median_tile(const MedianArgs &a, int bx, int by,
   const int tid = threadIdx.x;
   const int tx = tid & 15, ty = tid >> 4;
   const int x0 = a.f0 + bx, y0 = a.f1 + by;
+  constexpr int PB = PACK == 2 ? BYH : 0;
   switch (a.dt) {
   case IFM_U8:
-  case IFM_N0F8: median_stage<IFM_U8, LW, LH>(a, x0 - G::H, y0 - G::H, ibase, tile); break;
-  case IFM_F32: median_stage<IFM_F32, LW, LH>(a, x0 - G::H, y0 - G::H, ibase, tile); break;
-  case IFM_BF16: median_stage<IFM_BF16, LW, LH>(a, x0 - G::H, y0 - G::H, ibase, tile); break;
-  default: median_stage<IFM_F16, LW, LH>(a, x0 - G::H, y0 - G::H, ibase, tile); break;
+  case IFM_N0F8: median_stage<IFM_U8, LW, LH, PB>(a, x0 - G::H, y0 - G::H, ibase, tile); break;
+  case IFM_F32: median_stage<IFM_F32, LW, LH, 0>(a, x0 - G::H, y0 - G::H, ibase, tile); break;
+  case IFM_BF16: median_stage<IFM_BF16, LW, LH, PB>(a, x0 - G::H, y0 - G::H, ibase, tile); break;
+  default: median_stage<IFM_F16, LW, LH, PB>(a, x0 - G::H, y0 - G::H, ibase, tile); break;
   }
   __syncthreads();
   const int cx = tx * TX, cy = ty * TY;
@@ -131,10 +142,7 @@ __device__ __forceinline__ void median_tile(const MedianArgs &a, int bx, int by,
     for (int r = 0; r < VH; ++r)
 #pragma unroll
       for (int c = 0; c < VW; ++c) {
-        u16x2 p;
-        p.x = (uint16_t)tile[(cy + r) * LW + cx + c];
-        p.y = (uint16_t)tile[(cy + BYH + r) * LW + cx + c];
-        v[c + VW * r] = p;
+        v[c + VW * r] = __builtin_bit_cast(u16x2, tile[(cy + r) * LW + cx + c]);
       }
     u16x2 o[TX * TY];
     MedianNet<K, TX, TY>::run(v, o);

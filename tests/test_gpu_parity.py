@@ -553,6 +553,31 @@ def test_median_float_isless_and_generic_windows():
     np.testing.assert_array_equal(IF.mapwindow(IF.median_, v, 5), O.median(v, [-2], [5], "replicate"))
 
 
+@pytest.mark.parametrize("et", ["bf16", "f16"])
+def test_median_16bit_floats_packed_tiles(et):
+    # 16-bit keys run two tiles per register (packed staging): several tiles
+    # in both directions, NaN and signed zeros, every border
+    rng = np.random.default_rng(24)
+    x = rng.standard_normal((150, 141)).astype(np.float32)
+    x[5, 7] = np.nan
+    x[70, 71] = -0.0
+    x[70, 72] = 0.0
+    if et == "bf16":
+        a = IF.bf16(x)
+        xv = IF.bf16_to_float32(a)
+    else:
+        a = x.astype(np.float16)
+        xv = a.astype(np.float32)
+    for w in (3, 5, 7):
+        h = w // 2
+        for b in STYLES + [IF.Fill(0.5)]:
+            want = O.median(xv.astype(np.float64), [-h, -h], [w, w], obord(b)).astype(np.float32)
+            got = IF.mapwindow(IF.median_, a, (w, w), border=b)
+            gv = IF.bf16_to_float32(got) if et == "bf16" else np.asarray(got).astype(np.float32)
+            np.testing.assert_array_equal(gv, want, err_msg="%d %r" % (w, b))
+            assert np.array_equal(np.signbit(gv), np.signbit(want))
+
+
 def test_median_reference_vectors():
     b = np.array([1, 100, 1, 2, -1000, 2], dtype=np.float64)
     np.testing.assert_array_equal(IF.mapwindow(IF.median_, b, range(-1, 2)), [1, 1, 2, 1, 2, 2])

@@ -771,8 +771,39 @@ __global__ void __launch_bounds__(256) k_minmax2d(Minmax2dArgs a) {
   const int gx0 = a.f0 + ox0 + a.wf0, gy0 = a.f1 + oy0 + a.wf1;
   for (int plane = blockIdx.z; plane < a.nplanes; plane += gridDim.z) {
     const int64_t ibase = (int64_t)plane * a.in_plane, obase = (int64_t)plane * a.out_plane;
-    // Stage: the thread's column, every other row, 8 loads in flight.
-    {
+    if constexpr (W0 > 0 && W1 > 0) {
+      // Stage: lane c owns columns c and c + 64, wave w rows w, w + 4, ...
+      // (a row's clamp is wave-uniform); every load is issued before the
+      // first LDS write.
+      constexpr int IW = MM_TX + W0 - 1, IH = MM_TY + W1 - 1, NR = (IH + 3) / 4;
+      static_assert(IW <= 128, "two columns per lane");
+      const int c = tid & 63;
+      const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+      int sx[2];
+      bool cok[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        cok[h] = c + 64 * h < IW;
+        sx[h] = min(max(gx0 + c + 64 * h, 0), a.n0 - 1);
+      }
+      uint32_t v[NR][2];
+#pragma unroll
+      for (int j = 0; j < NR; ++j) {
+        const int r = w + 4 * j;
+        const int sy = min(max(gy0 + r, 0), a.n1 - 1);
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+          if (r < IH && cok[h]) v[j][h] = key32(a.in, ibase + (int64_t)sy * a.n0 + sx[h], DT);
+      }
+#pragma unroll
+      for (int j = 0; j < NR; ++j) {
+        const int r = w + 4 * j;
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+          if (r < IH && cok[h]) s_in[r * iwp + c + 64 * h] = v[j][h];
+      }
+    } else {
+      // Stage: the thread's column, every other row, 8 loads in flight.
       const int c = tid & 127, rh = tid >> 7;
       const bool colok = c < iw;
       const int sx = min(max(gx0 + (colok ? c : iw - 1), 0), a.n0 - 1);

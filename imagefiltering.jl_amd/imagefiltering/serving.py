@@ -170,14 +170,15 @@ def stats(mix, done):
 
 
 def autotune(mix, budget_bytes=64 << 30, sample=3000, batches=(256, 512, 1024, 2048), inflight=(1, 2, 4, 7),
-             seed=20261208):
+             seed=20261208, slack=0.02):
     """Schedule autotuner for the request loop: runs closed-loop trials of a
     seeded sample of the stream (requests drawn from `mix`, all arriving at
-    once) for every (max_batch, max_inflight) pair and returns the pair with the
-    highest images/s, with the trial table.  Each trial reuses the same sample,
-    so the pairs are compared on identical work.  max_inflight is capped at 7
-    by the request loop (eight descriptor slots, one kept free for the batch
-    being built)."""
+    once) for every (max_batch, max_inflight) pair and returns, among the pairs
+    within `slack` of the highest images/s, the one with the fewest batches in
+    flight and then the smallest batches (shallower queues wait less), with the
+    trial table.  Each trial reuses the same sample, so the pairs are compared
+    on identical work.  max_inflight is capped at 7 by the request loop (eight
+    descriptor slots, one kept free for the batch being built)."""
     n = mix["n"]
     rng = np.random.default_rng(seed)
     idx = np.sort(rng.choice(n, size=min(sample, n), replace=False))
@@ -190,5 +191,7 @@ def autotune(mix, budget_bytes=64 << 30, sample=3000, batches=(256, 512, 1024, 2
         for mi in inflight:
             _, done = serve(sub, "batched", budget_bytes=budget_bytes, max_batch=mb, max_inflight=mi)
             table.append({"max_batch": mb, "max_inflight": mi, "images_per_s": stats(sub, done)["images_per_s"]})
-    best = max(table, key=lambda r: r["images_per_s"] or 0.0)
+    top = max(r["images_per_s"] or 0.0 for r in table)
+    near = [r for r in table if (r["images_per_s"] or 0.0) >= (1.0 - slack) * top]
+    best = min(near, key=lambda r: (r["max_inflight"], r["max_batch"]))
     return {"max_batch": best["max_batch"], "max_inflight": best["max_inflight"]}, table

@@ -194,7 +194,11 @@ def test_schedule_autotuner_picks_a_tried_schedule():
     assert len(table) == 4
     assert {(r["max_batch"], r["max_inflight"]) for r in table} == {(8, 1), (8, 3), (32, 1), (32, 3)}
     best = max(r["images_per_s"] for r in table)
-    assert [r for r in table if (r["max_batch"], r["max_inflight"]) == (sched["max_batch"], sched["max_inflight"])][0][
-        "images_per_s"] == best
+    chosen = [r for r in table if (r["max_batch"], r["max_inflight"]) == (sched["max_batch"], sched["max_inflight"])][0]
+    assert chosen["images_per_s"] >= 0.98 * best
+    # no pair within the slack has fewer batches in flight, or as few and smaller batches
+    for r in table:
+        if r["images_per_s"] >= 0.98 * best:
+            assert (r["max_inflight"], r["max_batch"]) >= (chosen["max_inflight"], chosen["max_batch"])
     admit, done = serving.serve(mix, "batched", budget_bytes=1 << 30, **sched)
     assert np.all(done >= admit)

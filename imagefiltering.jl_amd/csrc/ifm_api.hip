@@ -246,6 +246,7 @@ int ifm_ctx_destroy(ifm_ctx *ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   for (auto &s : ctx->slots)
     if (s.first) (void)hipFree(s.first);
+  if (ctx->fft_work) (void)hipFree(ctx->fft_work);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   for (auto &e : ctx->taps) {
@@ -260,6 +261,7 @@ int ifm_ctx_destroy(ifm_ctx *ctx) {
 int ifm_ctx_set_stream(ifm_ctx *ctx, void *stream) {
   if (!ctx) return IFM_E_ARG;
   ctx->stream = (hipStream_t)stream;
+  ctx->fft_key = 0;  // the kept kernel spectrum was written on the previous stream
   return IFM_OK;
 }
 
@@ -685,11 +687,42 @@ int ifm_imfilter_fft(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const in
       if (N[m] > 8192) return fail(ctx, IFM_E_UNSUPPORTED, "FFT length above 8192 on one axis");
     }
   }
+  // The FFT workspace has its own buffer: besides the transforms it keeps the
+  // last kernel's spectrum, reused while the kernel and padded sizes repeat.
   size_t ws = fft_workspace(N, 3) + dk.taps.size() * sizeof(float) + 256;
-  void *work = scratch(ctx, S_WORK, ws);
-  if (!work) return fail(ctx, IFM_E_NOMEM, "device allocation failed (FFT workspace)");
-  if (vo.d.count() > 0)
-    HIPCHK(launch_fft_filter(vi.d, vo.d, of, dk, N, nd, lo, b->style, (float)b->fill, work, ctx->stream));
+  if (ctx->fft_bytes < ws) {
+    if (ctx->fft_work) {
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+      HIPCHK(hipFree(ctx->fft_work));
+    }
+    ctx->fft_work = nullptr;
+    ctx->fft_bytes = 0;
+    ctx->fft_key = 0;
+    if (hipMalloc(&ctx->fft_work, ws + ws / 8) != hipSuccess) {
+      ctx->fft_work = nullptr;
+      return fail(ctx, IFM_E_NOMEM, "device allocation failed (FFT workspace)");
+    }
+    ctx->fft_bytes = ws + ws / 8;
+  }
+  uint64_t key = 1469598103934665603ull;  // FNV-1a over taps, kernel shape and offsets, padded sizes
+  {
+    const unsigned char *p = (const unsigned char *)dk.taps.data();
+    for (size_t i = 0; i < dk.taps.size() * sizeof(float); ++i) key = (key ^ p[i]) * 1099511628211ull;
+    for (int d = 0; d < MAXD; ++d) {
+      key = (key ^ (uint64_t)dk.n[d]) * 1099511628211ull;
+      key = (key ^ (uint64_t)(dk.first[d] + 0x40000000)) * 1099511628211ull;
+    }
+    for (int d = 0; d < 3; ++d) key = (key ^ (uint64_t)N[d]) * 1099511628211ull;
+    key = (key ^ (uint64_t)nd) * 1099511628211ull;
+    if (key == 0) key = 1;
+  }
+  const bool have_kernel = ctx->fft_key == key;
+  ctx->fft_key = 0;  // valid again once this call's kernel spectrum is in place
+  if (vo.d.count() > 0) {
+    HIPCHK(launch_fft_filter(vi.d, vo.d, of, dk, N, nd, lo, b->style, (float)b->fill, ctx->fft_work, have_kernel,
+                             ctx->stream));
+    ctx->fft_key = key;
+  }
   if ((rc = finish_view(ctx, vo))) return rc;
   return sync_if_host(ctx, !in->device || !out->device);
 }

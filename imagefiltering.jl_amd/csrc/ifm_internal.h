@@ -280,9 +280,11 @@ size_t minmax_workspace(const Dense4 &in, const int64_t *osize);
 
 // FFT correlation (up to 3-D).
 size_t fft_workspace(const int64_t *N, int nd);
+// have_kernel: the workspace still holds this kernel's spectrum and twiddles
+// from the previous call with the same kernel and sizes.
 hipError_t launch_fft_filter(const Dense4 &in, Dense4 &out, const int64_t *out_first, const DenseKernel &k,
                              const int64_t *Npad, int nd, const int64_t *lo, int style, float fill, void *work,
-                             hipStream_t st);
+                             bool have_kernel, hipStream_t st);
 
 // Batched serving.  Each kernel takes an array of per-request descriptors in
 // device memory, the request of every tile (tile_req) and the first tile of
@@ -338,6 +340,10 @@ struct ifm_ctx {
   std::vector<BatchSlot> bslots;
   int bnext = 0;
   std::vector<ifm_tile_rule> tile_rules;  // ifm_set_tile_rules
+  // FFT workspace (its own slot) and the key of the kernel spectrum it holds
+  void *fft_work = nullptr;
+  size_t fft_bytes = 0;
+  uint64_t fft_key = 0;
 };
 
 namespace ifm {

@@ -286,89 +286,151 @@ size_t fft_workspace(const int64_t *N, int nd) {
   return (size_t)tot * sizeof(float2) * 3 + tw * sizeof(float2) + 256;  // + kernel taps (caller)
 }
 
-static hipError_t fft_all_axes(float2 *x, float2 *tmp, const int *N, int nd, float2 *const *tw, int sign,
-                               hipStream_t st) {
-  int n3[3] = {N[0], nd > 1 ? N[1] : 1, nd > 2 ? N[2] : 1};
-  for (int ax = 0; ax < nd; ++ax) {
-    FftPlan plan;
-    make_plan(N[ax], plan);
-    FftLinesArgs la;
-    la.tw = tw[ax];
-    la.N = N[ax];
-    la.sign = sign;
-    la.npass = plan.npass;
-    for (int p = 0; p < plan.npass; ++p) la.radix[p] = plan.radix[p];
-    int64_t total = (int64_t)n3[0] * n3[1] * n3[2];
-    la.nlines = total / N[ax];
-    size_t lds = sizeof(float2) * (size_t)N[ax];
-    unsigned blocks = (unsigned)(la.nlines < 65536 ? la.nlines : 65536);
-    if (ax == 0) {
-      la.data = x;
-      hipLaunchKernelGGL(k_fft_lines, dim3(blocks), dim3(FFT_THREADS), lds, st, la);
-    } else {
-      TransArgs t;
-      t.in = x;
-      t.out = tmp;
-      t.n0 = n3[0];
-      t.n1 = n3[1];
-      t.n2 = n3[2];
-      t.ax = ax;
-      int na = ax == 1 ? n3[1] : n3[2], nm = ax == 1 ? n3[2] : n3[1];
-      dim3 g((n3[0] + 31) / 32, (na + 31) / 32, nm < 65535 ? nm : 65535);
-      hipLaunchKernelGGL(k_transpose, g, dim3(256), 0, st, t);
-      la.data = tmp;
-      hipLaunchKernelGGL(k_fft_lines, dim3(blocks), dim3(FFT_THREADS), lds, st, la);
-      // transpose back: the swapped array has dims (na, ., n0) in the same pattern
-      TransArgs b;
-      b.in = tmp;
-      b.out = x;
-      if (ax == 1) { b.n0 = n3[1]; b.n1 = n3[0]; b.n2 = n3[2]; }
-      else { b.n0 = n3[2]; b.n1 = n3[1]; b.n2 = n3[0]; }
-      b.ax = ax;
-      int nb = ax == 1 ? b.n1 : b.n2, mb = ax == 1 ? b.n2 : b.n1;
-      dim3 g2((b.n0 + 31) / 32, (nb + 31) / 32, mb < 65535 ? mb : 65535);
-      hipLaunchKernelGGL(k_transpose, g2, dim3(256), 0, st, b);
-    }
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+// Transforms along axis `ax` of the n3 array in x.  ax == 0: in place.
+// Otherwise x is transposed (axis 0 <-> ax) into tmp and tmp's lines are
+// transformed; with `keep` the result stays in tmp in that swapped layout,
+// else it is transposed back into x.
+static void fft_lines(float2 *data, int n, int64_t nlines, const float2 *tw, int sign, hipStream_t st) {
+  FftPlan plan;
+  make_plan(n, plan);
+  FftLinesArgs la;
+  la.data = data;
+  la.tw = tw;
+  la.N = n;
+  la.sign = sign;
+  la.npass = plan.npass;
+  for (int p = 0; p < plan.npass; ++p) la.radix[p] = plan.radix[p];
+  la.nlines = nlines;
+  const size_t lds = sizeof(float2) * (size_t)n;
+  const unsigned blocks = (unsigned)(nlines < 65536 ? nlines : 65536);
+  hipLaunchKernelGGL(k_fft_lines, dim3(blocks), dim3(FFT_THREADS), lds, st, la);
+}
+
+// Swaps axis 0 with axis ax (1 or 2) of the n0 x n1 x n2 array in into out.
+static void transpose3(const float2 *in, float2 *out, int n0, int n1, int n2, int ax, hipStream_t st) {
+  TransArgs t;
+  t.in = in;
+  t.out = out;
+  t.n0 = n0;
+  t.n1 = n1;
+  t.n2 = n2;
+  t.ax = ax;
+  const int na = ax == 1 ? n1 : n2, nm = ax == 1 ? n2 : n1;
+  dim3 g((n0 + 31) / 32, (na + 31) / 32, nm < 65535 ? nm : 65535);
+  hipLaunchKernelGGL(k_transpose, g, dim3(256), 0, st, t);
+}
+
+// Dimensions of the n3 array with axes 0 and ax swapped.
+static void swapped(const int *n3, int ax, int *s3) {
+  s3[0] = n3[ax];
+  s3[1] = ax == 1 ? n3[0] : n3[1];
+  s3[2] = ax == 2 ? n3[0] : n3[2];
+}
+
+static void fft_axis(float2 *x, float2 *tmp, const int *n3, int ax, const float2 *tw, int sign, bool keep,
+                     hipStream_t st) {
+  const int64_t total = (int64_t)n3[0] * n3[1] * n3[2];
+  if (ax == 0) {
+    fft_lines(x, n3[0], total / n3[0], tw, sign, st);
+    return;
   }
-  return hipSuccess;
+  transpose3(x, tmp, n3[0], n3[1], n3[2], ax, st);
+  fft_lines(tmp, n3[ax], total / n3[ax], tw, sign, st);
+  if (!keep) {
+    int s3[3];
+    swapped(n3, ax, s3);
+    transpose3(tmp, x, s3[0], s3[1], s3[2], ax, st);
+  }
+}
+
+// Forward transform of every axis of src; the last axis (if not 0) is left
+// in its swapped layout in tmp.  Returns the buffer holding the spectrum.
+static float2 *fft_forward_keep(float2 *src, float2 *tmp, const int *n3, int nd, float2 *const *tw, hipStream_t st) {
+  for (int ax = 0; ax < nd; ++ax) fft_axis(src, tmp, n3, ax, tw[ax], -1, ax == nd - 1, st);
+  return nd > 1 ? tmp : src;
+}
+
+// Inverse of fft_forward_keep: spec holds the spectrum in that layout, other
+// is the second buffer; returns the buffer holding the (unscaled) result in
+// the natural layout.
+static float2 *fft_inverse_keep(float2 *spec, float2 *other, const int *n3, int nd, float2 *const *tw,
+                                hipStream_t st) {
+  if (nd == 1) {
+    fft_lines(spec, n3[0], (int64_t)n3[0] * n3[1] * n3[2] / n3[0], tw[0], +1, st);
+    return spec;
+  }
+  const int last = nd - 1;
+  const int64_t total = (int64_t)n3[0] * n3[1] * n3[2];
+  fft_lines(spec, n3[last], total / n3[last], tw[last], +1, st);
+  int s3[3];
+  swapped(n3, last, s3);
+  transpose3(spec, other, s3[0], s3[1], s3[2], last, st);
+  for (int ax = 0; ax < last; ++ax) fft_axis(other, spec, n3, ax, tw[ax], +1, false, st);
+  return other;
 }
 
 hipError_t launch_fft_filter(const Dense4 &in, Dense4 &out, const int64_t *out_first, const DenseKernel &k,
                              const int64_t *Npad, int nd, const int64_t *lo, int style, float fill, void *work,
-                             hipStream_t st) {
+                             bool have_kernel, hipStream_t st) {
   // Axes 0 .. nd-1 are transformed (each N a product of 2s and 3s); further
-  // axes are batch axes with N = n.
+  // axes are batch axes with N = n.  Work buffers A, B, C of N0 N1 N2 complex
+  // values, then the twiddle tables, then the kernel taps.  The kernel's
+  // spectrum is left in C and reused by the next call with the same kernel
+  // and sizes (have_kernel); the image uses A and B only.
   int N[3] = {(int)Npad[0], (int)Npad[1], (int)Npad[2]};
+  int n3[3] = {N[0], nd > 1 ? N[1] : 1, nd > 2 ? N[2] : 1};
   int64_t tot = (int64_t)N[0] * N[1] * N[2];
-  float2 *x = (float2 *)work;
-  float2 *kx = x + tot;
-  float2 *tmp = kx + tot;
-  float2 *twbase = tmp + tot;
-  // twiddle tables (computed in FP64 on the host)
+  float2 *A = (float2 *)work;
+  float2 *B = A + tot;
+  float2 *Cb = B + tot;
+  float2 *twbase = Cb + tot;
   float2 *tw[3];
   size_t ntw = 0;
-  for (int d = 0; d < nd; ++d) ntw += (size_t)N[d];
-  std::vector<float2> h(ntw);
-  float2 *tp = twbase;
-  size_t at = 0;
   for (int d = 0; d < nd; ++d) {
-    for (int m = 0; m < N[d]; ++m) {
-      double ang = -2.0 * M_PI * (double)m / (double)N[d];
-      h[at + m] = make_float2((float)cos(ang), (float)sin(ang));
-    }
-    tw[d] = tp;
-    tp += N[d];
-    at += N[d];
+    tw[d] = twbase + ntw;
+    ntw += (size_t)N[d];
   }
-  {
-    hipError_t e = hipMemcpyAsync(twbase, h.data(), sizeof(float2) * ntw, hipMemcpyHostToDevice, st);
-    if (e != hipSuccess) return e;
+  float *taps_dev = (float *)(twbase + ntw);
+  hipError_t e;
+  if (!have_kernel) {
+    // twiddle tables (computed in FP64 on the host) and the kernel taps
+    std::vector<float2> h(ntw);
+    size_t at = 0;
+    for (int d = 0; d < nd; ++d) {
+      for (int m = 0; m < N[d]; ++m) {
+        double ang = -2.0 * M_PI * (double)m / (double)N[d];
+        h[at + m] = make_float2((float)cos(ang), (float)sin(ang));
+      }
+      at += N[d];
+    }
+    if ((e = hipMemcpyAsync(twbase, h.data(), sizeof(float2) * ntw, hipMemcpyHostToDevice, st)) != hipSuccess)
+      return e;
+    if ((e = hipMemcpyAsync(taps_dev, k.taps.data(), sizeof(float) * k.taps.size(), hipMemcpyHostToDevice, st)) !=
+        hipSuccess)
+      return e;
+    // the host buffers must outlive their copies
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+    // the kernel, wrapped into the padded array (FFTView), starts in B (nd > 1)
+    // or C (nd == 1) so that its spectrum ends in C
+    float2 *ks = nd > 1 ? B : Cb;
+    if ((e = hipMemsetAsync(ks, 0, sizeof(float2) * tot, st)) != hipSuccess) return e;
+    FftKernArgs ka;
+    ka.taps = taps_dev;
+    ka.x = ks;
+    ka.ktotal = 1;
+    for (int d = 0; d < 3; ++d) {
+      ka.kn[d] = d < k.ndim ? k.n[d] : 1;
+      ka.kf[d] = d < k.ndim ? k.first[d] : 0;
+      ka.N[d] = N[d];
+      ka.ktotal *= ka.kn[d];
+    }
+    hipLaunchKernelGGL(k_fft_kernel_scatter, dim3(grid_for(ka.ktotal)), dim3(256), 0, st, ka);
+    fft_forward_keep(ks, Cb, n3, nd, tw, st);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   FftFillArgs fa;
   fa.in = in.data;
-  fa.x = x;
+  fa.x = A;
   for (int d = 0; d < 3; ++d) {
     fa.n[d] = (int)in.n[d];
     fa.N[d] = N[d];
@@ -379,28 +441,11 @@ hipError_t launch_fft_filter(const Dense4 &in, Dense4 &out, const int64_t *out_f
   fa.fill = fill;
   fa.total = tot;
   hipLaunchKernelGGL(k_fft_fill, dim3(grid_for(tot)), dim3(256), 0, st, fa);
-  hipError_t e = hipMemsetAsync(kx, 0, sizeof(float2) * tot, st);
-  if (e != hipSuccess) return e;
-  // kernel taps were uploaded after the twiddles
-  FftKernArgs ka;
-  ka.taps = (const float *)tp;
-  ka.x = kx;
-  ka.ktotal = 1;
-  for (int d = 0; d < 3; ++d) {
-    ka.kn[d] = d < k.ndim ? k.n[d] : 1;
-    ka.kf[d] = d < k.ndim ? k.first[d] : 0;
-    ka.N[d] = N[d];
-    ka.ktotal *= ka.kn[d];
-  }
-  e = hipMemcpyAsync((void *)tp, k.taps.data(), sizeof(float) * k.taps.size(), hipMemcpyHostToDevice, st);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_fft_kernel_scatter, dim3(grid_for(ka.ktotal)), dim3(256), 0, st, ka);
-  if ((e = fft_all_axes(x, tmp, N, nd, tw, -1, st)) != hipSuccess) return e;
-  if ((e = fft_all_axes(kx, tmp, N, nd, tw, -1, st)) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_cmul_conj, dim3(grid_for(tot)), dim3(256), 0, st, x, (const float2 *)kx, tot);
-  if ((e = fft_all_axes(x, tmp, N, nd, tw, +1, st)) != hipSuccess) return e;
+  float2 *spec = fft_forward_keep(A, B, n3, nd, tw, st);
+  hipLaunchKernelGGL(k_cmul_conj, dim3(grid_for(tot)), dim3(256), 0, st, spec, (const float2 *)Cb, tot);
+  float2 *res = fft_inverse_keep(spec, spec == A ? B : A, n3, nd, tw, st);
   FftExtractArgs xa;
-  xa.x = x;
+  xa.x = res;
   xa.out = out.data;
   for (int d = 0; d < 3; ++d) {
     xa.o[d] = (int)out.n[d];
@@ -411,9 +456,7 @@ hipError_t launch_fft_filter(const Dense4 &in, Dense4 &out, const int64_t *out_f
   xa.dt = out.dtype;
   xa.total = out.count();
   hipLaunchKernelGGL(k_fft_extract, dim3(grid_for(xa.total)), dim3(256), 0, st, xa);
-  hipError_t le = hipGetLastError();
-  (void)hipStreamSynchronize(st);  // the host twiddle buffer must outlive its copy
-  return le;
+  return hipGetLastError();
 }
 
 }  // namespace ifm

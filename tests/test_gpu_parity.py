@@ -430,6 +430,19 @@ def test_fft_matches_fir(K):
     assert IF.filter_algorithm(img, IF.factorkernel(kern)) == IF.Algorithm.FFT()
 
 
+def test_fft_kernel_spectrum_reuse():
+    # the library keeps the last kernel's spectrum: repeat it over different
+    # images, switch kernels and padded sizes in between, and check every call
+    rng = np.random.default_rng(25)
+    k1, k2 = norm_random_kernel(rng, 15), norm_random_kernel(rng, 17)
+    imgs = [rng.random(s).astype(np.float32) for s in [(120, 97), (120, 97), (64, 130), (120, 97)]]
+    for i, (img, k) in enumerate(zip(imgs + imgs, [k1, k1, k1, k2, k1, k2, k2, k1])):
+        kern = IF.centered(k)
+        out = IF.imfilter(img, kern, "reflect", IF.Algorithm.FFT())
+        want = O.imfilter_dense(img.astype(np.float64), k, list(kern.first), "reflect")
+        assert relerr(out, want) <= TOL, i
+
+
 def test_fft_1d_3d():
     rng = np.random.default_rng(20)
     v = rng.random(500)

@@ -255,7 +255,16 @@ def run_extras(IF, _lib, L, ctx, T, img, d_img, d_out, b_reflect, args):
     rate("dense31_mfma_4096", lambda: L.ifm_imfilter_dense(ctx, a_in, a_out, None, ks31, b_reflect, 0), n * n, 8)
     k15 = IF.centered(norm_kernel(np.random.default_rng(20261214), 15))
     ks15 = _kernel_struct(k15)
+    # repeated kernel: its spectrum stays in the FFT workspace between calls
     rate("dense15_fft_4096", lambda: L.ifm_imfilter_fft(ctx, a_in, a_out, None, ks15, b_reflect), n * n, 8, 3)
+    # a new kernel every call (two kernels alternating): the kernel's transform is redone
+    k15b = _kernel_struct(IF.centered(norm_kernel(np.random.default_rng(20261216), 15)))
+    alt = [0]
+
+    def fft_alternating():
+        alt[0] ^= 1
+        return L.ifm_imfilter_fft(ctx, a_in, a_out, None, ks15 if alt[0] else k15b, b_reflect)
+    rate("dense15_fft_4096_new_kernel_each_call", fft_alternating, n * n, 8, 4)
     rate("dense15_valu_fp32_4096", lambda: L.ifm_imfilter_dense(ctx, a_in, a_out, None, ks15, b_reflect, 1), n * n, 8, 3)
     # BF16 storage in and out
     xb = IF.DeviceArray.from_host(IF.bf16(img))
@@ -309,8 +318,9 @@ def run_extras(IF, _lib, L, ctx, T, img, d_img, d_out, b_reflect, args):
 
     # 512 x 512 x 256 volume: separable FIR (symmetric, circular) and IIR (circular), F64 out
     vrng = np.random.default_rng(20261205)
-    vol = (vrng.random((512, 512, 256), dtype=np.float32)
-           + vrng.normal(0, 0.05, (512, 512, 256)).astype(np.float32))
+    # column-major like a Julia Array: dimension 1 (512) is the contiguous one
+    vol = np.asfortranarray(vrng.random((512, 512, 256), dtype=np.float32)
+                            + vrng.normal(0, 0.05, (512, 512, 256)).astype(np.float32))
     d_vol = IF.DeviceArray.from_host(vol)
     del vol
     o_vol = d_vol.similar(IF.Float64)

@@ -279,18 +279,20 @@ struct IirSegTables {
 
 // LDS (doubles): tables h, g (6 IS_S), then forward and backward end states
 // [wave][component][lane] (2 x nw x 3 x 64), then (lines along the contiguous
-// axis) one transpose buffer per wave of IS_TR lines x IS_TP.
-constexpr int IS_TR = 16;          // lines per transpose round (lanes 16 r .. 16 r + 15 in round r)
-constexpr int IS_TP = IS_S + 1;    // transpose buffer pitch (doubles): conflict-free both ways
+// axis) one transpose buffer per wave of 32 lines x IS_TP doubles, which also
+// holds 64 lines x IS_TP floats.  At nw = 8 the total is 160,768 bytes.
+constexpr int IS_TP = IS_S + 1;    // transpose buffer pitch (elements): conflict-free both ways
+constexpr int IS_TB = 32 * IS_TP;  // transpose buffer per wave (doubles)
 __host__ __device__ inline size_t iir_seg_lds_doubles(int nw, bool rows) {
-  return 6 * IS_S + 2 * (size_t)nw * 3 * 64 + (rows ? (size_t)nw * IS_TR * IS_TP : 0);
+  return 6 * IS_S + 2 * (size_t)nw * 3 * 64 + (rows ? (size_t)nw * IS_TB : 0);
 }
 
-// ROWS: lines along the contiguous axis.  A line's segment is then IS_S
-// consecutive elements, so a wave moves it through its transpose buffer in
-// rounds of IS_TR lines: lanes run along the segment for the global loads
-// and stores (one contiguous run of the line per instruction), and along the
-// lines for the recursion.
+// ROWS: lines along the contiguous axis (F32 or F64 input).  A line's
+// segment is then IS_S consecutive elements, so a wave moves it through its
+// transpose buffer: lanes run along the segment for the global loads and
+// stores (one contiguous run of a line per instruction), and along the lines
+// for the recursion.  F32 input moves all 64 lines in one round (float
+// buffer), F64 input and the F64 output in two rounds of 32 lines.
 template <int DT, bool ROWS>
 __global__ void __launch_bounds__(64 * IS_MAXW) k_iir_seg(IirArgs p, IirSegTables tb) {
   extern __shared__ double sm[];
@@ -308,23 +310,28 @@ __global__ void __launch_bounds__(64 * IS_MAXW) k_iir_seg(IirArgs p, IirSegTable
   const int64_t st = p.stride;
   const int64_t base = (line % st) + (line / st) * st * n + (int64_t)s0 * st;
   double v[IS_S];
-  double *tr = sm + iir_seg_lds_doubles(nw, false) + w * IS_TR * IS_TP;  // ROWS only
+  double *tr = sm + iir_seg_lds_doubles(nw, false) + w * IS_TB;  // ROWS only
   if constexpr (ROWS) {
+    static_assert(DT == IFM_F32 || DT == IFM_F64, "typed contiguous lines only");
+    using TE = typename std::conditional<DT == IFM_F32, float, double>::type;
+    constexpr int TRL = DT == IFM_F32 ? 64 : 32;  // lines per round
+    TE *trl = (TE *)tr;
+    const TE *src = (const TE *)p.src;
     const int64_t line0 = (int64_t)blockIdx.x * 64;
-    for (int r = 0; r < 64 / IS_TR; ++r) {
-      double x[IS_TR];
+    for (int r = 0; r < 64 / TRL; ++r) {
+      TE x[TRL];
 #pragma unroll
-      for (int i = 0; i < IS_TR; ++i) {
-        const int64_t ln = line0 + r * IS_TR + i;
-        x[i] = (lane < L && ln < p.lines) ? iir_ld<DT>(p, ln * n + s0 + lane) : 0.0;
+      for (int i = 0; i < TRL; ++i) {
+        const int64_t ln = line0 + r * TRL + i;
+        x[i] = (lane < L && ln < p.lines) ? src[ln * n + s0 + lane] : (TE)0;
       }
 #pragma unroll
-      for (int i = 0; i < IS_TR; ++i) tr[i * IS_TP + lane] = x[i];
+      for (int i = 0; i < TRL; ++i) trl[i * IS_TP + lane] = x[i];
       __syncthreads();
-      if ((lane >> 4) == r) {
+      if (lane / TRL == r) {
 #pragma unroll
         for (int j = 0; j < IS_S; ++j)
-          if (j < L) v[j] = tr[(lane & (IS_TR - 1)) * IS_TP + j];
+          if (j < L) v[j] = (double)trl[(lane % TRL) * IS_TP + j];
       }
       __syncthreads();
     }
@@ -435,16 +442,16 @@ __global__ void __launch_bounds__(64 * IS_MAXW) k_iir_seg(IirArgs p, IirSegTable
   const double sc = p.scale;
   if constexpr (ROWS) {
     const int64_t line0 = (int64_t)blockIdx.x * 64;
-    for (int r = 0; r < 64 / IS_TR; ++r) {
-      if ((lane >> 4) == r) {
+    for (int r = 0; r < 2; ++r) {
+      if ((lane >> 5) == r) {
 #pragma unroll
         for (int j = 0; j < IS_S; ++j)
-          if (j < L) tr[(lane & (IS_TR - 1)) * IS_TP + j] = v[j] * sc;
+          if (j < L) tr[(lane & 31) * IS_TP + j] = v[j] * sc;
       }
       __syncthreads();
 #pragma unroll
-      for (int i = 0; i < IS_TR; ++i) {
-        const int64_t ln = line0 + r * IS_TR + i;
+      for (int i = 0; i < 32; ++i) {
+        const int64_t ln = line0 + r * 32 + i;
         if (lane < L && ln < p.lines) p.dst[ln * n + s0 + lane] = tr[i * IS_TP + lane];
       }
       __syncthreads();

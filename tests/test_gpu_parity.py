@@ -168,11 +168,12 @@ def test_separable_3d_and_1d():
 
 
 def test_separable_3d_fused_many_tiles():
-    # Several (x, y) tiles, two z chunks with a partial last chunk, z extent not
-    # a multiple of the axis-2 factor length; F32 and F64 input; every border.
+    # Several (x, y) tiles, three z chunks of 64 planes with a partial last
+    # chunk, z extent not a multiple of the axis-2 factor length; F32 and F64
+    # input; every border.
     rng = np.random.default_rng(41)
     for dt, sig in [(np.float32, (1.0, 2.0, 1.0)), (np.float64, (2.0, 1.0, 2.0))]:
-        vol = np.asfortranarray(rng.random((70, 45, 43)).astype(dt))
+        vol = np.asfortranarray(rng.random((70, 45, 139)).astype(dt))
         kern = KernelFactors.gaussian(sig)
         for b in ["replicate", "reflect", "symmetric", "circular", IF.Fill(0.5)]:
             out = IF.imfilter(vol, kern, b)

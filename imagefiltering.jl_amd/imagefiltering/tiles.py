@@ -87,13 +87,19 @@ def save(result, path=None):
 
 def load(path=None):
     """Install the rules of a saved tuning result; returns it (None if there
-    is no cache file)."""
+    is no cache file, or if it cannot be read: the default tiles then stay,
+    with a warning)."""
     path = path or cache_path()
     if not os.path.exists(path):
         return None
-    with open(path) as f:
-        result = json.load(f)
-    install(result["rules"])
+    try:
+        with open(path) as f:
+            result = json.load(f)
+        install(result["rules"])
+    except Exception as e:  # a stale or damaged cache must not break filtering
+        import warnings
+        warnings.warn("ignoring the tile cache %s: %s" % (path, e))
+        return None
     return result
 
 

@@ -115,3 +115,16 @@ def test_tall_tiles_batched_bit_identical(restore_rules):
         np.testing.assert_array_equal(res[64][i][0], res[32][i][0], err_msg="request %d" % i)
         if res[32][i][1] is not None:
             np.testing.assert_array_equal(res[64][i][1], res[32][i][1])
+
+
+def test_damaged_cache_is_ignored(tmp_path, monkeypatch):
+    p = tmp_path / "tiles.json"
+    p.write_text("{not json")
+    installed = []
+    monkeypatch.setattr(tiles, "install", lambda rules: installed.append(rules))
+    with pytest.warns(UserWarning):
+        assert tiles.load(str(p)) is None
+    p.write_text('{"rules": [{"op": 1, "dtype": 2, "kmax": 9, "tile_h": 64, "max_pixels": 100}]}')
+    assert tiles.load(str(p))["rules"][0]["tile_h"] == 64
+    assert installed and installed[-1][0]["kmax"] == 9
+    assert tiles.load(str(tmp_path / "absent.json")) is None

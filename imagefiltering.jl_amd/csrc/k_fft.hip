@@ -15,7 +15,7 @@
 namespace ifm {
 
 constexpr int FFT_MAXN = 8192;
-constexpr int FFT_THREADS = 256;
+constexpr int FFT_THREADS = 512;  // 8 waves per line: 7% faster than 4 (occupancy), 1024 slower
 constexpr int FFT_MAXJ = FFT_MAXN / (2 * FFT_THREADS);  // radix-2 butterflies per thread
 constexpr int FFT_MAXPASS = 16;
 

@@ -200,19 +200,20 @@ struct FftFillArgs {
   int64_t total;
 };
 
+// Grid: x over axis 0 of the padded array, y over its rows (axes 1 and 2);
+// a row's border maps are computed once per row.
 __global__ void __launch_bounds__(256) k_fft_fill(FftFillArgs a) {
-  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < a.total;
-       q += (int64_t)gridDim.x * blockDim.x) {
-    int64_t r = q;
-    int u0 = (int)(r % a.N[0]); r /= a.N[0];
-    int u1 = (int)(r % a.N[1]); r /= a.N[1];
-    int u2 = (int)r;
-    int s0 = bmap(a.style, u0 - a.lo[0], a.n[0]);
-    int s1 = bmap(a.style, u1 - a.lo[1], a.n[1]);
-    int s2 = bmap(a.style, u2 - a.lo[2], a.n[2]);
-    float v = (s0 | s1 | s2) < 0 ? a.fill
-                                 : load_f(a.in, s0 + (int64_t)a.n[0] * (s1 + (int64_t)a.n[1] * s2), a.dt);
-    a.x[q] = make_float2(v, 0.f);
+  const int u0 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u0 >= a.N[0]) return;
+  const int s0 = bmap(a.style, u0 - a.lo[0], a.n[0]);
+  const int nrows = a.N[1] * a.N[2];
+  for (int row = blockIdx.y; row < nrows; row += gridDim.y) {
+    const int u1 = row % a.N[1], u2 = row / a.N[1];
+    const int s1 = bmap(a.style, u1 - a.lo[1], a.n[1]);
+    const int s2 = bmap(a.style, u2 - a.lo[2], a.n[2]);
+    const float v = (s0 | s1 | s2) < 0 ? a.fill
+                                       : load_f(a.in, s0 + (int64_t)a.n[0] * (s1 + (int64_t)a.n[1] * s2), a.dt);
+    a.x[(int64_t)row * a.N[0] + u0] = make_float2(v, 0.f);
   }
 }
 
@@ -258,15 +259,15 @@ struct FftExtractArgs {
   int64_t total;
 };
 
+// Grid: x over axis 0 of the output, y over its rows (axes 1 and 2).
 __global__ void __launch_bounds__(256) k_fft_extract(FftExtractArgs a) {
-  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < a.total;
-       q += (int64_t)gridDim.x * blockDim.x) {
-    int64_t r = q;
-    int c0 = (int)(r % a.o[0]); r /= a.o[0];
-    int c1 = (int)(r % a.o[1]); r /= a.o[1];
-    int c2 = (int)r;
-    int64_t idx = (c0 + a.off[0]) + (int64_t)a.N[0] * ((c1 + a.off[1]) + (int64_t)a.N[1] * (c2 + a.off[2]));
-    store_f(a.out, q, a.dt, a.x[idx].x * a.scale);
+  const int c0 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c0 >= a.o[0]) return;
+  const int nrows = a.o[1] * a.o[2];
+  for (int row = blockIdx.y; row < nrows; row += gridDim.y) {
+    const int c1 = row % a.o[1], c2 = row / a.o[1];
+    const int64_t idx = (c0 + a.off[0]) + (int64_t)a.N[0] * ((c1 + a.off[1]) + (int64_t)a.N[1] * (c2 + a.off[2]));
+    store_f(a.out, (int64_t)row * a.o[0] + c0, a.dt, a.x[idx].x * a.scale);
   }
 }
 
@@ -440,7 +441,10 @@ hipError_t launch_fft_filter(const Dense4 &in, Dense4 &out, const int64_t *out_f
   fa.dt = in.dtype;
   fa.fill = fill;
   fa.total = tot;
-  hipLaunchKernelGGL(k_fft_fill, dim3(grid_for(tot)), dim3(256), 0, st, fa);
+  {
+    const int rows = N[1] * N[2];
+    hipLaunchKernelGGL(k_fft_fill, dim3((N[0] + 255) / 256, rows < 65535 ? rows : 65535), dim3(256), 0, st, fa);
+  }
   float2 *spec = fft_forward_keep(A, B, n3, nd, tw, st);
   hipLaunchKernelGGL(k_cmul_conj, dim3(grid_for(tot)), dim3(256), 0, st, spec, (const float2 *)Cb, tot);
   float2 *res = fft_inverse_keep(spec, spec == A ? B : A, n3, nd, tw, st);
@@ -455,7 +459,11 @@ hipError_t launch_fft_filter(const Dense4 &in, Dense4 &out, const int64_t *out_f
   xa.scale = (float)(1.0 / (double)tot);
   xa.dt = out.dtype;
   xa.total = out.count();
-  hipLaunchKernelGGL(k_fft_extract, dim3(grid_for(xa.total)), dim3(256), 0, st, xa);
+  if (xa.total > 0) {
+    const int rows = xa.o[1] * xa.o[2];
+    hipLaunchKernelGGL(k_fft_extract, dim3((xa.o[0] + 255) / 256, rows < 65535 ? rows : 65535), dim3(256), 0, st,
+                       xa);
+  }
   return hipGetLastError();
 }
 

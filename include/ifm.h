@@ -147,7 +147,9 @@ int ifm_imfilter_dense(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const 
 /* ---- imfilter: FFT ------------------------------------------------------- */
 /* Replaces _imfilter_fft! / filtfft (src/imfilter.jl:796-866) with padding
  * to nextprod([2,3], n + K) per axis as Pad(kernel, img, FFT())
- * (src/border.jl:240-248).  Up to 3 dims. */
+ * (src/border.jl:240-248).  Up to 3 dims.  The ctx keeps the last kernel's
+ * spectrum and reuses it while the kernel and the padded sizes repeat
+ * (ifm_ctx_set_stream drops it). */
 int ifm_imfilter_fft(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const int64_t *out_first,
                      const ifm_kernel *kernel, const ifm_border *border);
 

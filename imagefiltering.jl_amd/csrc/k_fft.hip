@@ -271,6 +271,13 @@ __global__ void __launch_bounds__(256) k_fft_extract(FftExtractArgs a) {
   }
 }
 
+// Row blocks of a (x blocks, rows) launch: about 4096 workgroups in all, each
+// looping over its rows, rather than one short workgroup per row.
+static unsigned rows_grid(int rows, int xb) {
+  const int want = 4096 / (xb > 0 ? xb : 1);
+  return (unsigned)(rows < want ? rows : (want > 0 ? want : 1));
+}
+
 static unsigned grid_for(int64_t total) {
   int64_t b = (total + 255) / 256;
   return (unsigned)(b > (1 << 20) ? (1 << 20) : (b < 1 ? 1 : b));
@@ -442,8 +449,8 @@ hipError_t launch_fft_filter(const Dense4 &in, Dense4 &out, const int64_t *out_f
   fa.fill = fill;
   fa.total = tot;
   {
-    const int rows = N[1] * N[2];
-    hipLaunchKernelGGL(k_fft_fill, dim3((N[0] + 255) / 256, rows < 65535 ? rows : 65535), dim3(256), 0, st, fa);
+    const int rows = N[1] * N[2], xb = (N[0] + 255) / 256;
+    hipLaunchKernelGGL(k_fft_fill, dim3(xb, rows_grid(rows, xb)), dim3(256), 0, st, fa);
   }
   float2 *spec = fft_forward_keep(A, B, n3, nd, tw, st);
   hipLaunchKernelGGL(k_cmul_conj, dim3(grid_for(tot)), dim3(256), 0, st, spec, (const float2 *)Cb, tot);
@@ -460,9 +467,8 @@ hipError_t launch_fft_filter(const Dense4 &in, Dense4 &out, const int64_t *out_f
   xa.dt = out.dtype;
   xa.total = out.count();
   if (xa.total > 0) {
-    const int rows = xa.o[1] * xa.o[2];
-    hipLaunchKernelGGL(k_fft_extract, dim3((xa.o[0] + 255) / 256, rows < 65535 ? rows : 65535), dim3(256), 0, st,
-                       xa);
+    const int rows = xa.o[1] * xa.o[2], xb = (xa.o[0] + 255) / 256;
+    hipLaunchKernelGGL(k_fft_extract, dim3(xb, rows_grid(rows, xb)), dim3(256), 0, st, xa);
   }
   return hipGetLastError();
 }

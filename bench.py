@@ -141,7 +141,10 @@ def main():
         from imagefiltering import tiles
         t0 = time.time()
         tt = tiles.tune()
-        tiles.save(tt)
+        try:
+            tiles.save(tt)
+        except OSError as e:  # an unwritable cache location only loses the cache
+            log("bench: tile cache not saved: %s" % e)
         tile_tune = {"seconds": round(time.time() - t0, 2),
                      "rules": [[r["dtype"], r["kmax"], r["max_pixels"], r["tile_h"]] for r in tt["rules"]],
                      "rule_fields": ["dtype", "kmax", "max_pixels", "tile_h"], "trials": tt["trials"]}

@@ -28,7 +28,7 @@ int fail(ifm_ctx *ctx, int code, const std::string &msg) {
     if (e_ != hipSuccess) return fail(ctx, IFM_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-void *scratch(ifm_ctx *ctx, int slot, size_t bytes) {
+void *slot_buffer(ifm_ctx *ctx, int slot, size_t bytes) {
   if (ctx->slots.size() < S_COUNT) ctx->slots.resize(S_COUNT, {nullptr, 0});
   auto &s = ctx->slots[slot];
   if (bytes == 0) bytes = 16;
@@ -126,7 +126,7 @@ int make_view(ifm_ctx *ctx, const ifm_array *a, const Geo &g, int slot, bool upl
     v.staged = false;
     return IFM_OK;
   }
-  void *p = scratch(ctx, slot, v.bytes);
+  void *p = slot_buffer(ctx, slot, v.bytes);
   if (!p) return fail(ctx, IFM_E_NOMEM, "device allocation failed (staging)");
   if (upload && v.bytes) HIPCHK(hipMemcpyAsync(p, a->data, v.bytes, hipMemcpyHostToDevice, ctx->stream));
   v.d.data = p;
@@ -192,6 +192,8 @@ bool is_float_out(int dt) { return dt == IFM_F32 || dt == IFM_F64 || dt == IFM_B
 
 }  // namespace
 
+
+
 namespace ifm {
 
 int sep2d_tile_h(const ifm_ctx *ctx, int dt_in, int k0, int k1, int64_t pixels) {
@@ -253,6 +255,13 @@ int ifm_ctx_destroy(ifm_ctx *ctx) {
     (void)hipFree(e.dev);
     if (e.table) (void)hipFree(e.table);
   }
+  for (auto &e : ctx->tile_tw) {
+    (void)hipFree(e.tw32);
+    (void)hipFree(e.tw64);
+  }
+  for (auto &e : ctx->tile_spec) (void)hipFree(e.spec);
+  if (ctx->tile_taps) (void)hipFree(ctx->tile_taps);
+  if (ctx->tile_flag) (void)hipFree(ctx->tile_flag);
   batch_release(ctx);
   delete ctx;
   return IFM_OK;
@@ -260,8 +269,10 @@ int ifm_ctx_destroy(ifm_ctx *ctx) {
 
 int ifm_ctx_set_stream(ifm_ctx *ctx, void *stream) {
   if (!ctx) return IFM_E_ARG;
+  // cached kernel spectra and tap tables were written on the previous stream
+  if (ctx->stream != (hipStream_t)stream) HIPCHK(hipStreamSynchronize(ctx->stream));
   ctx->stream = (hipStream_t)stream;
-  ctx->fft_key = 0;  // the kept kernel spectrum was written on the previous stream
+  ctx->fft_have = false;
   return IFM_OK;
 }
 
@@ -375,9 +386,9 @@ int ifm_padarray(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const ifm_bo
 }
 
 // ---- FIR, separable ------------------------------------------------------------
-int ifm_imfilter_fir(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const int64_t *out_first,
-                     const ifm_factor *factors, int nfactors, const ifm_border *b) {
-  if (!ctx) return IFM_E_ARG;
+// The factor cascade; temporaries come from the ctx's slots.
+static int fir_run(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const int64_t *out_first,
+                   const ifm_factor *factors, int nfactors, const ifm_border *b) {
   Geo g;
   int rc;
   if ((rc = common_setup(ctx, in, out, g))) return rc;
@@ -485,7 +496,7 @@ int ifm_imfilter_fir(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const in
             dst.n[m] = later ? vi.d.n[m] : vo.d.n[m];
             dst_first[m] = later ? 0 : of[m];
           }
-          dst.data = scratch(ctx, (gi & 1) ? S_TMP2 : S_TMP1, (size_t)dst.count() * sizeof(float));
+          dst.data = slot_buffer(ctx, (gi & 1) ? S_TMP2 : S_TMP1, (size_t)dst.count() * sizeof(float));
           if (!dst.data) return fail(ctx, IFM_E_NOMEM, "device allocation failed (temporary)");
         }
         if (groups[gi].size() == 2) {
@@ -508,6 +519,12 @@ int ifm_imfilter_fir(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const in
   return sync_if_host(ctx, !in->device || !out->device);
 }
 
+int ifm_imfilter_fir(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const int64_t *out_first,
+                     const ifm_factor *factors, int nfactors, const ifm_border *b) {
+  if (!ctx) return IFM_E_ARG;
+  return fir_run(ctx, in, out, out_first, factors, nfactors, b);
+}
+
 // ---- FIR, dense --------------------------------------------------------------------
 static int device_taps(ifm_ctx *ctx, const DenseKernel &dk, bool want_table, ifm_ctx::TapEntry **out) {
   const std::vector<float> &t = dk.taps;
@@ -517,7 +534,8 @@ static int device_taps(ifm_ctx *ctx, const DenseKernel &dk, bool want_table, ifm
   for (int d = 0; d < MAXD; ++d) h = (h ^ (uint64_t)dk.n[d]) * 1099511628211ull;
   ifm_ctx::TapEntry *hit = nullptr;
   for (auto &e : ctx->taps)
-    if (e.hash == h && e.n == t.size()) hit = &e;
+    if (e.hash == h && e.n == t.size() && e.kx == dk.n[0] && e.ky == dk.n[1] && e.kz == dk.n[2] * 1024 + dk.n[3] && e.host == t)
+      hit = &e;
   if (!hit) {
     if (ctx->taps.size() >= 64) {  // bounded cache: drop the oldest entry
       HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -533,6 +551,10 @@ static int device_taps(ifm_ctx *ctx, const DenseKernel &dk, bool want_table, ifm
     e.hash = h;
     e.n = t.size();
     e.dev = d;
+    e.host = t;
+    e.kx = dk.n[0];
+    e.ky = dk.n[1];
+    e.kz = dk.n[2] * 1024 + dk.n[3];  // axes 2 and 3 (each at most 1024 taps)
     ctx->taps.push_back(e);
     hit = &ctx->taps.back();
   }
@@ -634,6 +656,71 @@ static int64_t nextprod23(int64_t n) {
   return best;
 }
 
+// Twiddle tables of the tiled FFT for tile size F (uploaded once per ctx).
+static int tile_twiddles(ifm_ctx *ctx, int F, const float2 **tw32, const double2 **tw64) {
+  for (auto &e : ctx->tile_tw)
+    if (e.F == F) {
+      *tw32 = (const float2 *)e.tw32;
+      *tw64 = (const double2 *)e.tw64;
+      return IFM_OK;
+    }
+  std::vector<float2> h32;
+  std::vector<double2> h64;
+  fft_tile_twiddles(F, h32, h64);
+  ifm_ctx::TileTw e;
+  e.F = F;
+  if (hipMalloc(&e.tw32, sizeof(float2) * F) != hipSuccess) return fail(ctx, IFM_E_NOMEM, "device allocation failed (FFT twiddles)");
+  if (hipMalloc(&e.tw64, sizeof(double2) * F) != hipSuccess) {
+    (void)hipFree(e.tw32);
+    return fail(ctx, IFM_E_NOMEM, "device allocation failed (FFT twiddles)");
+  }
+  ctx->tile_tw.push_back(e);
+  HIPCHK(hipMemcpy(e.tw32, h32.data(), sizeof(float2) * F, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(e.tw64, h64.data(), sizeof(double2) * F, hipMemcpyHostToDevice));
+  *tw32 = (const float2 *)e.tw32;
+  *tw64 = (const double2 *)e.tw64;
+  return IFM_OK;
+}
+
+// The kernel's F x F spectrum for the tiled FFT, cached per ctx and keyed by
+// the exact FP32 taps, the kernel shape and F.
+static int tile_spectrum(ifm_ctx *ctx, const DenseKernel &dk, int F, const double2 *tw64, const float2 **spec) {
+  for (auto &e : ctx->tile_spec)
+    if (e.F == F && e.k0 == dk.n[0] && e.k1 == dk.n[1] && e.taps == dk.taps) {
+      *spec = (const float2 *)e.spec;
+      return IFM_OK;
+    }
+  if (ctx->tile_spec.size() >= 16) {  // bounded cache: drop the oldest entry
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipFree(ctx->tile_spec.front().spec));
+    ctx->tile_spec.erase(ctx->tile_spec.begin());
+  }
+  const size_t tb = dk.taps.size() * sizeof(float);
+  if (ctx->tile_taps_bytes < tb) {
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (ctx->tile_taps) HIPCHK(hipFree(ctx->tile_taps));
+    ctx->tile_taps = nullptr;
+    ctx->tile_taps_bytes = 0;
+    if (hipMalloc(&ctx->tile_taps, tb) != hipSuccess) return fail(ctx, IFM_E_NOMEM, "device allocation failed (FFT taps)");
+    ctx->tile_taps_bytes = tb;
+  }
+  ifm_ctx::TileSpec e;
+  e.F = F;
+  e.k0 = dk.n[0];
+  e.k1 = dk.n[1];
+  e.taps = dk.taps;
+  if (hipMalloc(&e.spec, fft_tile_spec_bytes(F)) != hipSuccess)
+    return fail(ctx, IFM_E_NOMEM, "device allocation failed (FFT kernel spectrum)");
+  ctx->tile_spec.push_back(std::move(e));
+  ifm_ctx::TileSpec &h = ctx->tile_spec.back();
+  // the entry keeps the host taps alive until it is dropped (after a stream sync)
+  HIPCHK(hipMemcpyAsync(ctx->tile_taps, h.taps.data(), tb, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(launch_fft_tile_spectrum((const float *)ctx->tile_taps, dk.n[0], dk.n[1], F, tw64, (float2 *)h.spec,
+                                  ctx->stream));
+  *spec = (const float2 *)h.spec;
+  return IFM_OK;
+}
+
 int ifm_imfilter_fft(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const int64_t *out_first,
                      const ifm_kernel *kernel, const ifm_border *b) {
   if (!ctx) return IFM_E_ARG;
@@ -650,46 +737,71 @@ int ifm_imfilter_fft(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const in
   int64_t of[MAXD];
   to_canon(g, out_first, 0, of);
   if ((rc = check_region(ctx, vi.d, vo.d, of))) return rc;
-  // Axes with a kernel extent are padded to nextprod([2,3], n + K) and
-  // transformed; a kernel with a single tap at 0 on an axis is the identity.
-  int64_t N[MAXD], lo[MAXD];
   bool padded[MAXD];
+  for (int m = 0; m < MAXD; ++m) padded[m] = dk.n[m] > 1 || dk.first[m] != 0;
+  if ((rc = check_reflect(ctx, b, vi.d, padded))) return rc;
+
+  // Tiled overlap-save transform: kernels of up to 80 taps on axes 0 and 1.
+  const int F = fft_tile_size(dk);
+  if (F > 0) {
+    if (vo.d.count() > 0) {
+      const float2 *tw32;
+      const double2 *tw64;
+      const float2 *spec;
+      if ((rc = tile_twiddles(ctx, F, &tw32, &tw64))) return rc;
+      if ((rc = tile_spectrum(ctx, dk, F, tw64, &spec))) return rc;
+      if (!ctx->tile_flag) {
+        if (hipMalloc(&ctx->tile_flag, sizeof(int)) != hipSuccess)
+          return fail(ctx, IFM_E_NOMEM, "device allocation failed (FFT flag)");
+        HIPCHK(hipMemsetAsync(ctx->tile_flag, 0, sizeof(int), ctx->stream));
+      }
+      if (++ctx->tile_gen <= 0) ctx->tile_gen = 1;
+      HIPCHK(launch_fft_tile(vi.d, vo.d, of, dk, F, spec, tw32, ctx->tile_flag, ctx->tile_gen, b->style,
+                             (float)b->fill, ctx->stream));
+    }
+    if ((rc = finish_view(ctx, vo))) return rc;
+    return sync_if_host(ctx, !in->device || !out->device);
+  }
+
+  // Whole-array transform.  Axes with a kernel extent are padded to
+  // nextprod([2,3], n + K) (Pad(kernel, img, FFT()), src/border.jl:240-248),
+  // lengthened where needed so that every output's kernel window stays inside
+  // the padded range: with lo = max(-first, 0) the window of output c reaches
+  // c + lo + first + K - 1, so N >= of + on + max(first, 0) + K - 1.
+  int64_t N[MAXD], lo[MAXD];
   for (int m = 0; m < MAXD; ++m) {
-    padded[m] = dk.n[m] > 1 || dk.first[m] != 0;
     if (padded[m]) {
-      int64_t last = dk.first[m] + dk.n[m] - 1;
       lo[m] = dk.first[m] < 0 ? -dk.first[m] : 0;
-      (void)last;
-      N[m] = nextprod23(vi.d.n[m] + dk.n[m]);
-      if (N[m] > 8192) return fail(ctx, IFM_E_UNSUPPORTED, "FFT length above 8192 on one axis");
+      const int64_t reach = of[m] + vo.d.n[m] + (dk.first[m] > 0 ? dk.first[m] : 0) + dk.n[m] - 1;
+      N[m] = nextprod23(std::max<int64_t>(vi.d.n[m] + dk.n[m], reach));
     } else {
       lo[m] = 0;
       N[m] = vi.d.n[m];
     }
   }
-  if (N[3] != 1) return fail(ctx, IFM_E_UNSUPPORTED, "FFT filtering supports up to 3 dimensions");
-  if ((rc = check_reflect(ctx, b, vi.d, padded))) return rc;
-  // The FFT kernel transforms every one of the first nd axes; unpadded axes
-  // between padded ones are transformed too (a length-n DFT round trip).
+  if (padded[3]) return fail(ctx, IFM_E_UNSUPPORTED, "FFT filtering transforms at most 3 dimensions");
+  // The transform covers axes 0 .. nd-1; unpadded axes between padded ones are
+  // transformed too (a length-n DFT round trip), padded if their length is
+  // not a product of 2s and 3s.  Axes nd.. are batch axes.
   int nd = 1;
   for (int m = 0; m < 3; ++m)
     if (padded[m]) nd = m + 1;
-  for (int m = 0; m < nd; ++m)
-    if (N[m] > 8192) return fail(ctx, IFM_E_UNSUPPORTED, "FFT length above 8192 on one axis");
   for (int m = 0; m < nd; ++m) {
     int64_t x = N[m];
     while (x % 2 == 0) x /= 2;
     while (x % 3 == 0) x /= 3;
-    if (x != 1) {  // an unpadded axis of awkward length inside the transformed block
+    if (x != 1) {
       N[m] = nextprod23(vi.d.n[m] + dk.n[m]);
       padded[m] = true;
       lo[m] = dk.first[m] < 0 ? -dk.first[m] : 0;
-      if (N[m] > 8192) return fail(ctx, IFM_E_UNSUPPORTED, "FFT length above 8192 on one axis");
     }
+    if (!fft_axis_supported(N[m])) return fail(ctx, IFM_E_UNSUPPORTED, "FFT length not supported on one axis");
   }
-  // The FFT workspace has its own buffer: besides the transforms it keeps the
-  // last kernel's spectrum, reused while the kernel and padded sizes repeat.
-  size_t ws = fft_workspace(N, 3) + dk.taps.size() * sizeof(float) + 256;
+  if ((double)N[0] * (double)N[1] * (double)N[2] >= 2147483648.0)
+    return fail(ctx, IFM_E_UNSUPPORTED, "FFT padded array above 2^31 elements per 3-D block");
+  // A 4th axis is a batch axis handled block by block.
+  int64_t N3[MAXD] = {N[0], N[1], N[2], 1};
+  size_t ws = fft_workspace(N3, nd) + dk.taps.size() * sizeof(float) + 256;
   if (ctx->fft_bytes < ws) {
     if (ctx->fft_work) {
       HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -697,31 +809,36 @@ int ifm_imfilter_fft(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const in
     }
     ctx->fft_work = nullptr;
     ctx->fft_bytes = 0;
-    ctx->fft_key = 0;
+    ctx->fft_have = false;
     if (hipMalloc(&ctx->fft_work, ws + ws / 8) != hipSuccess) {
       ctx->fft_work = nullptr;
       return fail(ctx, IFM_E_NOMEM, "device allocation failed (FFT workspace)");
     }
     ctx->fft_bytes = ws + ws / 8;
   }
-  uint64_t key = 1469598103934665603ull;  // FNV-1a over taps, kernel shape and offsets, padded sizes
-  {
-    const unsigned char *p = (const unsigned char *)dk.taps.data();
-    for (size_t i = 0; i < dk.taps.size() * sizeof(float); ++i) key = (key ^ p[i]) * 1099511628211ull;
-    for (int d = 0; d < MAXD; ++d) {
-      key = (key ^ (uint64_t)dk.n[d]) * 1099511628211ull;
-      key = (key ^ (uint64_t)(dk.first[d] + 0x40000000)) * 1099511628211ull;
-    }
-    for (int d = 0; d < 3; ++d) key = (key ^ (uint64_t)N[d]) * 1099511628211ull;
-    key = (key ^ (uint64_t)nd) * 1099511628211ull;
-    if (key == 0) key = 1;
-  }
-  const bool have_kernel = ctx->fft_key == key;
-  ctx->fft_key = 0;  // valid again once this call's kernel spectrum is in place
+  bool have_kernel = ctx->fft_have && ctx->fft_nd == nd && ctx->fft_taps == dk.taps;
+  for (int d = 0; d < MAXD && have_kernel; ++d)
+    have_kernel = ctx->fft_kn[d] == dk.n[d] && ctx->fft_kf[d] == dk.first[d] && ctx->fft_N[d] == N3[d];
+  ctx->fft_have = false;  // valid again once this call's kernel spectrum is in place
   if (vo.d.count() > 0) {
-    HIPCHK(launch_fft_filter(vi.d, vo.d, of, dk, N, nd, lo, b->style, (float)b->fill, ctx->fft_work, have_kernel,
-                             ctx->stream));
-    ctx->fft_key = key;
+    const size_t ies = dtype_size(vi.d.dtype), oes = dtype_size(vo.d.dtype);
+    for (int64_t c3 = 0; c3 < vo.d.n[3]; ++c3) {
+      Dense4 bi = vi.d, bo = vo.d;
+      bi.n[3] = bo.n[3] = 1;
+      bi.data = (char *)vi.d.data + (size_t)((c3 + of[3]) * vi.d.n[0] * vi.d.n[1] * vi.d.n[2]) * ies;
+      bo.data = (char *)vo.d.data + (size_t)(c3 * vo.d.n[0] * vo.d.n[1] * vo.d.n[2]) * oes;
+      HIPCHK(launch_fft_filter(bi, bo, of, dk, N3, nd, lo, b->style, (float)b->fill, ctx->fft_work, have_kernel,
+                               ctx->stream));
+      have_kernel = true;
+    }
+    ctx->fft_have = true;
+    ctx->fft_nd = nd;
+    ctx->fft_taps = dk.taps;
+    for (int d = 0; d < MAXD; ++d) {
+      ctx->fft_kn[d] = dk.n[d];
+      ctx->fft_kf[d] = dk.first[d];
+      ctx->fft_N[d] = N3[d];
+    }
   }
   if ((rc = finish_view(ctx, vo))) return rc;
   return sync_if_host(ctx, !in->device || !out->device);
@@ -778,7 +895,7 @@ int ifm_imfilter_iir(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const if
         if (last && vo.d.dtype == IFM_F64) {
           dst.data = vo.d.data;
         } else {
-          dst.data = scratch(ctx, (s & 1) ? S_TMP2 : S_TMP1, (size_t)cnt * sizeof(double));
+          dst.data = slot_buffer(ctx, (s & 1) ? S_TMP2 : S_TMP1, (size_t)cnt * sizeof(double));
           if (!dst.data) return fail(ctx, IFM_E_NOMEM, "device allocation failed (temporary)");
         }
         IirCoef c;
@@ -899,7 +1016,7 @@ static int minmax_common(ifm_ctx *ctx, const ifm_array *in, ifm_array *omin, ifm
   if (!lemire && (rc = window_edge_check(ctx, style, vi.d, r, of, wf, wl))) return rc;
   if (vi.d.count() == 0 && r.count() > 0) return fail(ctx, IFM_E_ARG, "empty image");
   size_t ws = minmax_workspace(vi.d, r.n);
-  void *work = scratch(ctx, S_WORK, ws);
+  void *work = slot_buffer(ctx, S_WORK, ws);
   if (!work) return fail(ctx, IFM_E_NOMEM, "device allocation failed (min/max workspace)");
   HIPCHK(launch_minmax(vi.d, omin ? &vmin.d : nullptr, omax ? &vmax.d : nullptr, of, wf, wl,
                        style == IFM_FILL ? 1 : 0, fill, work, ctx->stream));

@@ -278,13 +278,33 @@ hipError_t launch_minmax(const Dense4 &in, Dense4 *omin, Dense4 *omax, const int
                          hipStream_t st);
 size_t minmax_workspace(const Dense4 &in, const int64_t *osize);
 
-// FFT correlation (up to 3-D).
+// FFT correlation over the whole padded array (up to 3 transformed axes; axes
+// beyond nd are batch axes).  Lines longer than FFT_LINE_MAX are transformed
+// in two levels.  The workspace holds two complex arrays of the padded size,
+// the kernel spectrum (product of the transformed N), the twiddle tables and
+// the kernel taps.
+constexpr int FFT_LINE_MAX = 8192;
 size_t fft_workspace(const int64_t *N, int nd);
+// A padded axis length the whole-array transform can take (2^a 3^b, split in
+// two levels beyond FFT_LINE_MAX).
+bool fft_axis_supported(int64_t n);
 // have_kernel: the workspace still holds this kernel's spectrum and twiddles
 // from the previous call with the same kernel and sizes.
 hipError_t launch_fft_filter(const Dense4 &in, Dense4 &out, const int64_t *out_first, const DenseKernel &k,
                              const int64_t *Npad, int nd, const int64_t *lo, int style, float fill, void *work,
                              bool have_kernel, hipStream_t st);
+// Tiled (overlap-save) FFT correlation for 1-D / 2-D kernels of up to 80 taps
+// per axis: F x F tiles, F = fft_tile_size(k) (0: the kernel does not fit).
+int fft_tile_size(const DenseKernel &k);
+size_t fft_tile_spec_bytes(int F);
+void fft_tile_twiddles(int F, std::vector<float2> &tw32, std::vector<double2> &tw64);
+hipError_t launch_fft_tile_spectrum(const float *d_taps, int K0, int K1, int F, const double2 *tw64, float2 *spec,
+                                    hipStream_t st);
+// nanflag / gen: a non-finite input value (or Fill value) sets *nanflag = gen
+// and the whole output becomes NaN, as the whole-array transform gives.
+hipError_t launch_fft_tile(const Dense4 &in, Dense4 &out, const int64_t *out_first, const DenseKernel &k, int F,
+                           const float2 *spec, const float2 *tw32, int *nanflag, int gen, int style, float fill,
+                           hipStream_t st);
 
 // Batched serving.  Each kernel takes an array of per-request descriptors in
 // device memory, the request of every tile (tile_req) and the first tile of
@@ -325,7 +345,8 @@ struct ifm_ctx {
     void *dev;                // FP32 taps
     void *table = nullptr;    // MFMA Toeplitz fragments (built on first MFMA use)
     float kinv = 1.f;
-    int kx = 0, ky = 0;
+    int kx = 0, ky = 0, kz = 0;
+    std::vector<float> host;  // the taps, compared exactly on lookup
   };
   std::vector<TapEntry> taps;
   // Batched runs: a ring of descriptor staging slots (pinned host + device),
@@ -340,10 +361,34 @@ struct ifm_ctx {
   std::vector<BatchSlot> bslots;
   int bnext = 0;
   std::vector<ifm_tile_rule> tile_rules;  // ifm_set_tile_rules
-  // FFT workspace (its own slot) and the key of the kernel spectrum it holds
+  // FFT workspace (its own slot) and the kernel spectrum it holds: valid when
+  // fft_have is set and the kernel, its offsets and the padded sizes equal
+  // the stored ones exactly.
   void *fft_work = nullptr;
   size_t fft_bytes = 0;
-  uint64_t fft_key = 0;
+  bool fft_have = false;
+  std::vector<float> fft_taps;
+  int fft_kn[IFM_MAXDIM] = {0, 0, 0, 0}, fft_kf[IFM_MAXDIM] = {0, 0, 0, 0};
+  int64_t fft_N[IFM_MAXDIM] = {0, 0, 0, 0};
+  int fft_nd = 0;
+  // Tiled FFT: per tile size F, the twiddle tables (FP32 and FP64), and a
+  // small cache of kernel spectra keyed by the exact taps, shape and F.
+  struct TileTw {
+    int F = 0;
+    void *tw32 = nullptr;
+    void *tw64 = nullptr;
+  };
+  std::vector<TileTw> tile_tw;
+  struct TileSpec {
+    int F = 0, k0 = 0, k1 = 0;
+    std::vector<float> taps;
+    void *spec = nullptr;
+  };
+  std::vector<TileSpec> tile_spec;
+  void *tile_taps = nullptr;  // device staging of the taps being transformed
+  size_t tile_taps_bytes = 0;
+  int *tile_flag = nullptr;   // non-finite flag of the tile kernel
+  int tile_gen = 0;
 };
 
 namespace ifm {

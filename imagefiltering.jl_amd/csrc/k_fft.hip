@@ -240,11 +240,26 @@ __global__ void __launch_bounds__(256) k_fft_kernel_scatter(FftKernArgs a) {
   }
 }
 
-__global__ void __launch_bounds__(256) k_cmul_conj(float2 *x, const float2 *k, int64_t total) {
+// x[q] *= conj(k[q % ktot]): the kernel spectrum repeats over the batch axes.
+__global__ void __launch_bounds__(256) k_cmul_conj(float2 *x, const float2 *k, int64_t total, int64_t ktot) {
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < total;
        q += (int64_t)gridDim.x * blockDim.x) {
-    float2 a = x[q], b = k[q];
+    float2 a = x[q], b = k[q % ktot];
     x[q] = make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+  }
+}
+
+// Two-level transform of a line of n = P Q values: element (k1, q) of the
+// P x Q intermediate (k1 fastest) times W_n^(sign k1 q).
+__global__ void __launch_bounds__(256) k_fft_twiddle4(float2 *x, int P, int Q, int64_t total, const float2 *tw,
+                                                       int sign) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int k1 = (int)(i % P), q = (int)((i / P) % Q);
+    float2 w = tw[(int64_t)k1 * q];
+    if (sign > 0) w.y = -w.y;
+    const float2 a = x[i];
+    x[i] = make_float2(a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x);
   }
 }
 
@@ -283,21 +298,70 @@ static unsigned grid_for(int64_t total) {
   return (unsigned)(b > (1 << 20) ? (1 << 20) : (b < 1 ? 1 : b));
 }
 
-// Twiddle tables live in the workspace after the three complex buffers.
-size_t fft_workspace(const int64_t *N, int nd) {
-  int64_t tot = 1;
-  size_t tw = 0;
-  for (int d = 0; d < nd; ++d) {
-    tot *= N[d];
-    tw += (size_t)N[d];
-  }
-  return (size_t)tot * sizeof(float2) * 3 + tw * sizeof(float2) + 256;  // + kernel taps (caller)
+// Per-axis transform plan: a line of n <= FFT_LINE_MAX runs in one LDS pass;
+// a longer one (n = P Q, both <= FFT_LINE_MAX) runs in two levels and leaves
+// its spectrum in a permuted order (position k2 + Q k1 holds X[k1 + P k2]).
+// The image and the kernel pass through the same transform, so their product
+// is taken in that order and the inverse restores the natural one.
+struct AxisPlan {
+  int n = 1, P = 0, Q = 0;
+  size_t tw = 0, twp = 0, twq = 0;  // offsets of W_n, W_P, W_Q in the twiddle area
+};
+
+static bool is23(int64_t x) {
+  if (x < 1) return false;
+  while (x % 2 == 0) x /= 2;
+  while (x % 3 == 0) x /= 3;
+  return x == 1;
 }
 
-// Transforms along axis `ax` of the n3 array in x.  ax == 0: in place.
-// Otherwise x is transposed (axis 0 <-> ax) into tmp and tmp's lines are
-// transformed; with `keep` the result stays in tmp in that swapped layout,
-// else it is transposed back into x.
+// Twiddle entries of one axis; fills the split of a long line.
+static size_t axis_plan(int64_t n, AxisPlan &ap, size_t at) {
+  ap.n = (int)n;
+  ap.tw = at;
+  size_t cnt = (size_t)n;
+  ap.P = ap.Q = 0;
+  if (n > FFT_LINE_MAX) {
+    // the most balanced split into two 2^a 3^b factors <= FFT_LINE_MAX
+    int64_t best = 0;
+    for (int64_t p = 1; p <= FFT_LINE_MAX; ++p)
+      if (n % p == 0 && is23(p) && n / p <= FFT_LINE_MAX && (best == 0 || llabs(p * p - n) < llabs(best * best - n)))
+        best = p;
+    ap.P = (int)best;
+    ap.Q = best ? (int)(n / best) : 0;
+    ap.twp = at + cnt;
+    cnt += (size_t)ap.P;
+    ap.twq = at + cnt;
+    cnt += (size_t)ap.Q;
+  }
+  return cnt;
+}
+
+static size_t twiddle_count(const int64_t *N, int nd) {
+  size_t at = 0;
+  for (int d = 0; d < nd; ++d) {
+    AxisPlan ap;
+    at += axis_plan(N[d], ap, at);
+  }
+  return at;
+}
+
+bool fft_axis_supported(int64_t n) {
+  if (!is23(n)) return false;
+  if (n <= FFT_LINE_MAX) return true;
+  AxisPlan ap;
+  axis_plan(n, ap, 0);
+  return ap.P > 0;
+}
+
+// Workspace: image arrays A and B (the padded size), the kernel spectrum C
+// (the transformed axes only), the twiddle tables; the caller adds the taps.
+size_t fft_workspace(const int64_t *N, int nd) {
+  int64_t tot = N[0] * N[1] * N[2], ktot = 1;
+  for (int d = 0; d < nd; ++d) ktot *= N[d];
+  return (size_t)(2 * tot + ktot) * sizeof(float2) + twiddle_count(N, nd) * sizeof(float2) + 256;
+}
+
 static void fft_lines(float2 *data, int n, int64_t nlines, const float2 *tw, int sign, hipStream_t st) {
   FftPlan plan;
   make_plan(n, plan);
@@ -315,17 +379,44 @@ static void fft_lines(float2 *data, int n, int64_t nlines, const float2 *tw, int
 }
 
 // Swaps axis 0 with axis ax (1 or 2) of the n0 x n1 x n2 array in into out.
-static void transpose3(const float2 *in, float2 *out, int n0, int n1, int n2, int ax, hipStream_t st) {
+static void transpose3(const float2 *in, float2 *out, int n0, int n1, int64_t n2, int ax, hipStream_t st) {
   TransArgs t;
   t.in = in;
   t.out = out;
   t.n0 = n0;
   t.n1 = n1;
-  t.n2 = n2;
+  t.n2 = (int)n2;
   t.ax = ax;
-  const int na = ax == 1 ? n1 : n2, nm = ax == 1 ? n2 : n1;
-  dim3 g((n0 + 31) / 32, (na + 31) / 32, nm < 65535 ? nm : 65535);
+  const int64_t na = ax == 1 ? n1 : n2, nm = ax == 1 ? n2 : n1;
+  dim3 g((n0 + 31) / 32, (unsigned)((na + 31) / 32), (unsigned)(nm < 65535 ? nm : 65535));
   hipLaunchKernelGGL(k_transpose, g, dim3(256), 0, st, t);
+}
+
+// nlines contiguous lines of n values, in place in data (scratch: a buffer of
+// the same size).  sign -1: forward, natural -> plan order; +1: inverse,
+// plan order -> natural, unnormalised.
+static void line_fft(float2 *data, float2 *scratch, const AxisPlan &ap, int64_t nlines, const float2 *twbase,
+                     int sign, hipStream_t st) {
+  if (ap.n <= FFT_LINE_MAX) {
+    fft_lines(data, ap.n, nlines, twbase + ap.tw, sign, st);
+    return;
+  }
+  const int P = ap.P, Q = ap.Q;  // element q + Q p of a line
+  const int64_t total = (int64_t)ap.n * nlines;
+  const float2 *twn = twbase + ap.tw;
+  if (sign < 0) {
+    transpose3(data, scratch, Q, P, nlines, 1, st);         // index p + P q
+    fft_lines(scratch, P, (int64_t)Q * nlines, twbase + ap.twp, -1, st);  // k1 + P q
+    hipLaunchKernelGGL(k_fft_twiddle4, dim3(grid_for(total)), dim3(256), 0, st, scratch, P, Q, total, twn, -1);
+    transpose3(scratch, data, P, Q, nlines, 1, st);         // q + Q k1
+    fft_lines(data, Q, (int64_t)P * nlines, twbase + ap.twq, -1, st);     // k2 + Q k1
+  } else {
+    fft_lines(data, Q, (int64_t)P * nlines, twbase + ap.twq, +1, st);     // q + Q k1
+    transpose3(data, scratch, Q, P, nlines, 1, st);         // k1 + P q
+    hipLaunchKernelGGL(k_fft_twiddle4, dim3(grid_for(total)), dim3(256), 0, st, scratch, P, Q, total, twn, +1);
+    fft_lines(scratch, P, (int64_t)Q * nlines, twbase + ap.twp, +1, st);  // p + P q
+    transpose3(scratch, data, P, Q, nlines, 1, st);         // q + Q p
+  }
 }
 
 // Dimensions of the n3 array with axes 0 and ax swapped.
@@ -335,15 +426,19 @@ static void swapped(const int *n3, int ax, int *s3) {
   s3[2] = ax == 2 ? n3[0] : n3[2];
 }
 
-static void fft_axis(float2 *x, float2 *tmp, const int *n3, int ax, const float2 *tw, int sign, bool keep,
-                     hipStream_t st) {
+// Transform along axis ax of the n3 array in x.  ax == 0: in place (tmp is
+// scratch).  Otherwise x is transposed (axis 0 <-> ax) into tmp and tmp's
+// lines are transformed (x is then scratch); with `keep` the result stays in
+// tmp in that swapped layout, else it is transposed back into x.
+static void fft_axis(float2 *x, float2 *tmp, const int *n3, int ax, const AxisPlan &ap, const float2 *twbase,
+                     int sign, bool keep, hipStream_t st) {
   const int64_t total = (int64_t)n3[0] * n3[1] * n3[2];
   if (ax == 0) {
-    fft_lines(x, n3[0], total / n3[0], tw, sign, st);
+    line_fft(x, tmp, ap, total / n3[0], twbase, sign, st);
     return;
   }
   transpose3(x, tmp, n3[0], n3[1], n3[2], ax, st);
-  fft_lines(tmp, n3[ax], total / n3[ax], tw, sign, st);
+  line_fft(tmp, x, ap, total / n3[ax], twbase, sign, st);
   if (!keep) {
     int s3[3];
     swapped(n3, ax, s3);
@@ -351,65 +446,69 @@ static void fft_axis(float2 *x, float2 *tmp, const int *n3, int ax, const float2
   }
 }
 
-// Forward transform of every axis of src; the last axis (if not 0) is left
-// in its swapped layout in tmp.  Returns the buffer holding the spectrum.
-static float2 *fft_forward_keep(float2 *src, float2 *tmp, const int *n3, int nd, float2 *const *tw, hipStream_t st) {
-  for (int ax = 0; ax < nd; ++ax) fft_axis(src, tmp, n3, ax, tw[ax], -1, ax == nd - 1, st);
+// Forward transform of axes 0 .. nd-1 of src (further axes are batch axes);
+// the last transformed axis (if not 0) is left in its swapped layout in tmp.
+// Returns the buffer holding the spectrum.
+static float2 *fft_forward_keep(float2 *src, float2 *tmp, float2 *scratch0, const int *n3, int nd,
+                                const AxisPlan *ap, const float2 *twbase, hipStream_t st) {
+  for (int ax = 0; ax < nd; ++ax)
+    fft_axis(src, ax == 0 ? scratch0 : tmp, n3, ax, ap[ax], twbase, -1, ax == nd - 1, st);
   return nd > 1 ? tmp : src;
 }
 
 // Inverse of fft_forward_keep: spec holds the spectrum in that layout, other
 // is the second buffer; returns the buffer holding the (unscaled) result in
 // the natural layout.
-static float2 *fft_inverse_keep(float2 *spec, float2 *other, const int *n3, int nd, float2 *const *tw,
-                                hipStream_t st) {
+static float2 *fft_inverse_keep(float2 *spec, float2 *other, const int *n3, int nd, const AxisPlan *ap,
+                                const float2 *twbase, hipStream_t st) {
+  const int64_t total = (int64_t)n3[0] * n3[1] * n3[2];
   if (nd == 1) {
-    fft_lines(spec, n3[0], (int64_t)n3[0] * n3[1] * n3[2] / n3[0], tw[0], +1, st);
+    line_fft(spec, other, ap[0], total / n3[0], twbase, +1, st);
     return spec;
   }
   const int last = nd - 1;
-  const int64_t total = (int64_t)n3[0] * n3[1] * n3[2];
-  fft_lines(spec, n3[last], total / n3[last], tw[last], +1, st);
+  line_fft(spec, other, ap[last], total / n3[last], twbase, +1, st);
   int s3[3];
   swapped(n3, last, s3);
   transpose3(spec, other, s3[0], s3[1], s3[2], last, st);
-  for (int ax = 0; ax < last; ++ax) fft_axis(other, spec, n3, ax, tw[ax], +1, false, st);
+  for (int ax = 0; ax < last; ++ax) fft_axis(other, spec, n3, ax, ap[ax], twbase, +1, false, st);
   return other;
 }
 
 hipError_t launch_fft_filter(const Dense4 &in, Dense4 &out, const int64_t *out_first, const DenseKernel &k,
                              const int64_t *Npad, int nd, const int64_t *lo, int style, float fill, void *work,
                              bool have_kernel, hipStream_t st) {
-  // Axes 0 .. nd-1 are transformed (each N a product of 2s and 3s); further
-  // axes are batch axes with N = n.  Work buffers A, B, C of N0 N1 N2 complex
-  // values, then the twiddle tables, then the kernel taps.  The kernel's
-  // spectrum is left in C and reused by the next call with the same kernel
-  // and sizes (have_kernel); the image uses A and B only.
+  // Axes 0 .. nd-1 are transformed (each N a product of 2s and 3s); axes nd..2
+  // are batch axes with N = n.  Work buffers A and B hold the padded image
+  // (N0 N1 N2 complex values), C the kernel spectrum over the transformed axes,
+  // then the twiddle tables and the kernel taps.  The kernel's spectrum is left
+  // in C and reused by the next call with the same kernel and sizes
+  // (have_kernel); the image uses A and B only.
   int N[3] = {(int)Npad[0], (int)Npad[1], (int)Npad[2]};
-  int n3[3] = {N[0], nd > 1 ? N[1] : 1, nd > 2 ? N[2] : 1};
-  int64_t tot = (int64_t)N[0] * N[1] * N[2];
+  int n3[3] = {N[0], N[1], N[2]};
+  int nk3[3] = {N[0], nd > 1 ? N[1] : 1, nd > 2 ? N[2] : 1};
+  const int64_t tot = (int64_t)N[0] * N[1] * N[2];
+  const int64_t ktot = (int64_t)nk3[0] * nk3[1] * nk3[2];
   float2 *A = (float2 *)work;
   float2 *B = A + tot;
   float2 *Cb = B + tot;
-  float2 *twbase = Cb + tot;
-  float2 *tw[3];
+  float2 *twbase = Cb + ktot;
+  AxisPlan ap[3];
   size_t ntw = 0;
-  for (int d = 0; d < nd; ++d) {
-    tw[d] = twbase + ntw;
-    ntw += (size_t)N[d];
-  }
+  for (int d = 0; d < nd; ++d) ntw += axis_plan(N[d], ap[d], ntw);
   float *taps_dev = (float *)(twbase + ntw);
   hipError_t e;
   if (!have_kernel) {
     // twiddle tables (computed in FP64 on the host) and the kernel taps
     std::vector<float2> h(ntw);
-    size_t at = 0;
     for (int d = 0; d < nd; ++d) {
-      for (int m = 0; m < N[d]; ++m) {
-        double ang = -2.0 * M_PI * (double)m / (double)N[d];
-        h[at + m] = make_float2((float)cos(ang), (float)sin(ang));
-      }
-      at += N[d];
+      const size_t offs[3] = {ap[d].tw, ap[d].twp, ap[d].twq};
+      const int lens[3] = {ap[d].n, ap[d].P, ap[d].Q};
+      for (int w = 0; w < 3; ++w)
+        for (int m = 0; m < lens[w]; ++m) {
+          double ang = -2.0 * M_PI * (double)m / (double)lens[w];
+          h[offs[w] + m] = make_float2((float)cos(ang), (float)sin(ang));
+        }
     }
     if ((e = hipMemcpyAsync(twbase, h.data(), sizeof(float2) * ntw, hipMemcpyHostToDevice, st)) != hipSuccess)
       return e;
@@ -418,10 +517,10 @@ hipError_t launch_fft_filter(const Dense4 &in, Dense4 &out, const int64_t *out_f
       return e;
     // the host buffers must outlive their copies
     if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
-    // the kernel, wrapped into the padded array (FFTView), starts in B (nd > 1)
-    // or C (nd == 1) so that its spectrum ends in C
+    // the kernel, wrapped into the padded array over the transformed axes
+    // (FFTView), starts in B (nd > 1) or C (nd == 1) so that its spectrum ends in C
     float2 *ks = nd > 1 ? B : Cb;
-    if ((e = hipMemsetAsync(ks, 0, sizeof(float2) * tot, st)) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(ks, 0, sizeof(float2) * ktot, st)) != hipSuccess) return e;
     FftKernArgs ka;
     ka.taps = taps_dev;
     ka.x = ks;
@@ -429,11 +528,11 @@ hipError_t launch_fft_filter(const Dense4 &in, Dense4 &out, const int64_t *out_f
     for (int d = 0; d < 3; ++d) {
       ka.kn[d] = d < k.ndim ? k.n[d] : 1;
       ka.kf[d] = d < k.ndim ? k.first[d] : 0;
-      ka.N[d] = N[d];
+      ka.N[d] = nk3[d];
       ka.ktotal *= ka.kn[d];
     }
     hipLaunchKernelGGL(k_fft_kernel_scatter, dim3(grid_for(ka.ktotal)), dim3(256), 0, st, ka);
-    fft_forward_keep(ks, Cb, n3, nd, tw, st);
+    fft_forward_keep(ks, Cb, nd > 1 ? Cb : A, nk3, nd, ap, twbase, st);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   FftFillArgs fa;
@@ -452,9 +551,9 @@ hipError_t launch_fft_filter(const Dense4 &in, Dense4 &out, const int64_t *out_f
     const int rows = N[1] * N[2], xb = (N[0] + 255) / 256;
     hipLaunchKernelGGL(k_fft_fill, dim3(xb, rows_grid(rows, xb)), dim3(256), 0, st, fa);
   }
-  float2 *spec = fft_forward_keep(A, B, n3, nd, tw, st);
-  hipLaunchKernelGGL(k_cmul_conj, dim3(grid_for(tot)), dim3(256), 0, st, spec, (const float2 *)Cb, tot);
-  float2 *res = fft_inverse_keep(spec, spec == A ? B : A, n3, nd, tw, st);
+  float2 *spec = fft_forward_keep(A, B, B, n3, nd, ap, twbase, st);
+  hipLaunchKernelGGL(k_cmul_conj, dim3(grid_for(tot)), dim3(256), 0, st, spec, (const float2 *)Cb, tot, ktot);
+  float2 *res = fft_inverse_keep(spec, spec == A ? B : A, n3, nd, ap, twbase, st);
   FftExtractArgs xa;
   xa.x = res;
   xa.out = out.data;
@@ -463,7 +562,7 @@ hipError_t launch_fft_filter(const Dense4 &in, Dense4 &out, const int64_t *out_f
     xa.off[d] = (int)(out_first[d] + lo[d]);
     xa.N[d] = N[d];
   }
-  xa.scale = (float)(1.0 / (double)tot);
+  xa.scale = (float)(1.0 / (double)ktot);
   xa.dt = out.dtype;
   xa.total = out.count();
   if (xa.total > 0) {

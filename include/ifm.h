@@ -145,11 +145,17 @@ int ifm_imfilter_dense(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const 
                        const ifm_kernel *kernel, const ifm_border *border, int precision);
 
 /* ---- imfilter: FFT ------------------------------------------------------- */
-/* Replaces _imfilter_fft! / filtfft (src/imfilter.jl:796-866) with padding
- * to nextprod([2,3], n + K) per axis as Pad(kernel, img, FFT())
- * (src/border.jl:240-248).  Up to 3 dims.  The ctx keeps the last kernel's
- * spectrum and reuses it while the kernel and the padded sizes repeat
- * (ifm_ctx_set_stream drops it). */
+/* Replaces _imfilter_fft! / filtfft (src/imfilter.jl:796-866): the
+ * correlation of the border-extended image with the kernel, computed by FFT.
+ * Kernels spanning axes 0 and 1 only with at most 80 taps per axis run as
+ * tiled overlap-save transforms (64 x 64 or 128 x 128 tiles, one HBM read and
+ * write per pixel; further axes are batch axes).  Other kernels use a
+ * whole-array transform padded to nextprod([2,3], n + K) per axis as
+ * Pad(kernel, img, FFT()) (src/border.jl:240-248), up to 3 transformed axes
+ * (a 4th is a batch axis), lines beyond 8192 in two levels.  The ctx caches
+ * kernel spectra (compared exactly) and reuses them while the kernel and sizes
+ * repeat.  A non-finite input value makes every output NaN, as the
+ * whole-array transform does. */
 int ifm_imfilter_fft(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const int64_t *out_first,
                      const ifm_kernel *kernel, const ifm_border *border);
 

@@ -458,6 +458,109 @@ def test_fft_1d_3d():
     assert relerr(out, want) <= TOL
 
 
+def _fft_check(img, k, kfirst, border, label):
+    kern = IF.OffsetArray(k, *[range(f, f + n) for f, n in zip(kfirst, k.shape)])
+    out = IF.imfilter(img, kern, border, IF.Algorithm.FFT())
+    want = O.imfilter_dense(np.asarray(img, dtype=np.float64), k, list(kfirst), obord(border))
+    assert out.shape == want.shape
+    assert relerr(out, want) <= TOL, label
+
+
+@pytest.mark.parametrize("order", ["C", "F"])
+def test_fft_kernel_with_fewer_dims_than_image(order):
+    # every line and plane is filtered when the kernel spans fewer axes than
+    # the image: 41 taps run on the tiled path, 101 on the whole-array path
+    # (the other axes are batch axes there)
+    rng = np.random.default_rng(31)
+    img = np.asarray(rng.random((70, 90)).astype(np.float32), order=order)
+    for L in (41, 101):
+        k = rng.random(L)
+        k /= k.sum()
+        for shape, first in (((L, 1), (-(L // 2), 0)), ((1, L), (0, -(L // 2)))):
+            _fft_check(img, k.reshape(shape), first, "reflect", (order, L, shape))
+        # the default choice for maxlen > 30 is FFT
+        out = IF.imfilter(img, IF.centered(k.reshape(L, 1)), "replicate")
+        want = O.imfilter_dense(img.astype(np.float64), k.reshape(L, 1), [-(L // 2), 0], "replicate")
+        assert relerr(out, want) <= TOL
+    vol = np.asarray(rng.random((13, 40, 37)).astype(np.float32), order=order)
+    k15 = norm_random_kernel(rng, 15)
+    for kshape, first in (((1, 15, 15), (0, -7, -7)), ((15, 15, 1), (-7, -7, 0)), ((15, 1, 15), (-7, 0, -7))):
+        _fft_check(vol, k15.reshape(kshape), first, "symmetric", (order, kshape))
+    kl = rng.standard_normal((1, 91, 5))
+    _fft_check(vol, kl, (0, -45, -2), IF.Fill(0.5), (order, "91x5"))
+    _fft_check(vol, kl.reshape(5, 91, 1), (-2, -45, 0), "circular", (order, "5x91"))
+
+
+@pytest.mark.parametrize("border", STYLES + ["fill"])
+def test_fft_kernel_offsets_beyond_centre(border):
+    # kernels whose first tap lies at +2 or more: every output's window ends
+    # past the image, so the padding must reach first + K - 1 beyond it
+    rng = np.random.default_rng(32)
+    b = IF.Fill(-0.5) if border == "fill" else border
+    img = rng.random((61, 47)).astype(np.float32)
+    for k, first in ((rng.standard_normal((3, 5)), (5, 2)), (rng.standard_normal((31, 4)), (2, -6)),
+                     (rng.standard_normal((95, 3)), (5, 1)), (rng.standard_normal((3, 88)), (-1, 3))):
+        _fft_check(img, k, first, b, (first, k.shape))
+    v = rng.random(61)
+    _fft_check(v, rng.standard_normal(3), (5,), b, "1-D")
+
+
+def test_fft_long_axes():
+    # padded lengths above 8192 take the two-level line transform (whole-array
+    # path, kernels longer than the tiles take); the tiled path has no limit
+    rng = np.random.default_rng(33)
+    v = rng.random(9000)
+    k = rng.random(101)
+    k /= k.sum()
+    _fft_check(v, k, (-50,), "reflect", "1-D 9000")
+    img = rng.random((9000, 3)).astype(np.float32)
+    _fft_check(img, k.reshape(101, 1), (-50, 0), "symmetric", "9000x3 axis 0")
+    _fft_check(np.asfortranarray(img), k.reshape(101, 1), (-50, 0), IF.Fill(0.25), "9000x3 F axis 0")
+    img2 = rng.random((2, 8300)).astype(np.float32)
+    _fft_check(img2, rng.random((1, 41)), (0, -20), "replicate", "tiled 8300")
+
+
+def test_fft_4d():
+    rng = np.random.default_rng(34)
+    a = rng.random((3, 4, 20, 22)).astype(np.float32)
+    _fft_check(a, rng.standard_normal((1, 1, 9, 7)), (0, 0, -4, -3), "reflect", "4-D tiled")
+    _fft_check(a, rng.standard_normal((1, 3, 5, 4)), (0, -1, -2, -2), "circular", "4-D whole-array")
+
+
+def test_fft_nonfinite_gives_nan():
+    # explicit Algorithm.FFT() on non-finite input: the transform spreads the
+    # NaN over the whole output, as the reference's whole-array FFT does
+    rng = np.random.default_rng(35)
+    for bad in (np.nan, np.inf):
+        img = rng.random((150, 170)).astype(np.float32)
+        img[77, 12] = bad
+        out = IF.imfilter(img, IF.centered(norm_random_kernel(rng, 15)), "reflect", IF.Algorithm.FFT())
+        assert np.all(np.isnan(out))
+    img = rng.random((150, 170)).astype(np.float32)
+    out = IF.imfilter(img, IF.centered(norm_random_kernel(rng, 15)), "reflect", IF.Algorithm.FFT())
+    assert np.all(np.isfinite(out))
+
+
+@pytest.mark.parametrize("dtype", ["float32", "bfloat16", "n0f8", "float64"])
+def test_fft_tiled_eltypes(dtype):
+    rng = np.random.default_rng(36)
+    x = rng.random((140, 133))
+    if dtype == "bfloat16":
+        img = IF.bf16(x.astype(np.float32))
+        x64 = IF.eltypes.bf16_to_float32(img).astype(np.float64)
+    elif dtype == "n0f8":
+        codes = (x * 255).astype(np.uint8)
+        img = IF.n0f8(codes)
+        x64 = codes / 255.0
+    else:
+        img = x.astype(dtype)
+        x64 = img.astype(np.float64)
+    k = norm_random_kernel(rng, 17)
+    out = IF.imfilter(img, IF.centered(k), "symmetric", IF.Algorithm.FFT())
+    want = O.imfilter_dense(x64, k, [-8, -8], "symmetric")
+    assert relerr(np.asarray(out, dtype=np.float64), want) <= TOL
+
+
 # ---- IIR -------------------------------------------------------------------------------------
 def iir_spec(kern):
     return [(f.Npre, f.data.a, f.data.b, f.data.scale, f.data.M) for f in kern]

@@ -5,6 +5,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -192,7 +193,36 @@ bool is_float_out(int dt) { return dt == IFM_F32 || dt == IFM_F64 || dt == IFM_B
 
 }  // namespace
 
+// Scratch of ifm_imfilter_fir_noalloc: the cascade's FP32 temporaries (two
+// halves, ping-pong).  Each device holds its own copy for the life of the
+// process.  In host code `scratch` is only a symbol handle: kernels and copies
+// use the device address that hipGetSymbolAddress returns for the current
+// device (noalloc_scratch_of).
+static __device__ __align__(256) unsigned char scratch[IFM_NOALLOC_SCRATCH_BYTES];
 
+namespace {
+
+std::mutex g_scratch_mu;
+constexpr int SCRATCH_MAXDEV = 64;
+void *g_scratch_ptr[SCRATCH_MAXDEV];
+bool g_scratch_done[SCRATCH_MAXDEV];
+
+// The device address of `scratch` on `device` (current), looked up once per
+// device; nullptr if the lookup failed.
+void *noalloc_scratch_of(int device) {
+  if (device < 0 || device >= SCRATCH_MAXDEV) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_done[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_ptr[device] = e == hipSuccess ? p : nullptr;
+    g_scratch_done[device] = true;
+    if (e != hipSuccess) (void)hipGetLastError();  // the failure is reported by the entry point
+  }
+  return g_scratch_ptr[device];
+}
+
+}  // namespace
 
 namespace ifm {
 
@@ -238,6 +268,7 @@ int ifm_ctx_create(int device, void *stream, ifm_ctx **out) {
   ifm_ctx *ctx = new ifm_ctx();
   ctx->device = device;
   ctx->stream = (hipStream_t)stream;
+  ctx->noalloc_scratch = noalloc_scratch_of(device);
   ctx->slots.resize(S_COUNT, {nullptr, 0});
   *out = ctx;
   return IFM_OK;
@@ -386,9 +417,11 @@ int ifm_padarray(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const ifm_bo
 }
 
 // ---- FIR, separable ------------------------------------------------------------
-// The factor cascade; temporaries come from the ctx's slots.
+// The factor cascade.  fixed_tmp == nullptr: temporaries come from the ctx's
+// slots (allocated on demand); otherwise from fixed_tmp, two halves of
+// fixed_cap / 2 bytes each, and nothing is allocated.
 static int fir_run(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const int64_t *out_first,
-                   const ifm_factor *factors, int nfactors, const ifm_border *b) {
+                   const ifm_factor *factors, int nfactors, const ifm_border *b, void *fixed_tmp, size_t fixed_cap) {
   Geo g;
   int rc;
   if ((rc = common_setup(ctx, in, out, g))) return rc;
@@ -496,7 +529,16 @@ static int fir_run(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const int6
             dst.n[m] = later ? vi.d.n[m] : vo.d.n[m];
             dst_first[m] = later ? 0 : of[m];
           }
-          dst.data = slot_buffer(ctx, (gi & 1) ? S_TMP2 : S_TMP1, (size_t)dst.count() * sizeof(float));
+          const size_t need = (size_t)dst.count() * sizeof(float);
+          if (fixed_tmp) {
+            if (need > fixed_cap / 2)
+              return fail(ctx, IFM_E_BUDGET,
+                          "cascade temporary of " + std::to_string(need) + " bytes exceeds the no-alloc maximum of " +
+                              std::to_string(fixed_cap / 2) + " bytes");
+            dst.data = (char *)fixed_tmp + ((gi & 1) ? fixed_cap / 2 : 0);
+          } else {
+            dst.data = slot_buffer(ctx, (gi & 1) ? S_TMP2 : S_TMP1, need);
+          }
           if (!dst.data) return fail(ctx, IFM_E_NOMEM, "device allocation failed (temporary)");
         }
         if (groups[gi].size() == 2) {
@@ -522,7 +564,17 @@ static int fir_run(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const int6
 int ifm_imfilter_fir(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const int64_t *out_first,
                      const ifm_factor *factors, int nfactors, const ifm_border *b) {
   if (!ctx) return IFM_E_ARG;
-  return fir_run(ctx, in, out, out_first, factors, nfactors, b);
+  return fir_run(ctx, in, out, out_first, factors, nfactors, b, nullptr, 0);
+}
+
+int ifm_imfilter_fir_noalloc(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const int64_t *out_first,
+                             const ifm_factor *factors, int nfactors, const ifm_border *b) {
+  if (!ctx) return IFM_E_ARG;
+  if (!in || !out || !in->device || !out->device)
+    return fail(ctx, IFM_E_ARG, "ifm_imfilter_fir_noalloc takes device arrays only");
+  if (!ctx->noalloc_scratch)
+    return fail(ctx, IFM_E_DEVICE, "the no-alloc scratch array has no device address on this device");
+  return fir_run(ctx, in, out, out_first, factors, nfactors, b, ctx->noalloc_scratch, IFM_NOALLOC_SCRATCH_BYTES);
 }
 
 // ---- FIR, dense --------------------------------------------------------------------

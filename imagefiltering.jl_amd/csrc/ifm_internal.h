@@ -389,6 +389,8 @@ struct ifm_ctx {
   size_t tile_taps_bytes = 0;
   int *tile_flag = nullptr;   // non-finite flag of the tile kernel
   int tile_gen = 0;
+  // device address of the no-alloc scratch array on this ctx's device
+  void *noalloc_scratch = nullptr;
 };
 
 namespace ifm {

@@ -139,6 +139,19 @@ int ifm_padarray(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const ifm_bo
 int ifm_imfilter_fir(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const int64_t *out_first,
                      const ifm_factor *factors, int nfactors, const ifm_border *border);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): ifm_imfilter_fir with the same arguments and results, but
+ * the cascade's FP32 temporaries live in a fixed device array of
+ * IFM_NOALLOC_SCRATCH_BYTES (two halves, one per ping-pong temporary; each GPU
+ * holds its own copy until the process ends) and nothing is allocated.  in and
+ * out must be device arrays.  A cascade whose temporary exceeds half the array
+ * returns IFM_E_BUDGET.  Every call on one GPU shares that GPU's array: calls
+ * that overlap on one GPU (several ctx streams at once) race. */
+#define IFM_NOALLOC_SCRATCH_BYTES (64u << 20)
+int ifm_imfilter_fir_noalloc(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const int64_t *out_first,
+                             const ifm_factor *factors, int nfactors, const ifm_border *border);
+
 /* Replaces the dense N-D correlation __imfilter_inbounds!
  * (src/imfilter.jl:592-669) on the padded image. */
 int ifm_imfilter_dense(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const int64_t *out_first,

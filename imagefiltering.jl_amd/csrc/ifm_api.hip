@@ -246,6 +246,35 @@ int sep2d_tile_h(const ifm_ctx *ctx, int dt_in, int k0, int k1, int64_t pixels) 
 
 }  // namespace ifm
 
+// all(isfinite, a) for filter_algorithm (src/imfilter.jl:1200-1209): one
+// streaming read, a flag set by any lane that meets an Inf or NaN.
+__global__ void __launch_bounds__(256) k_all_finite(const void *p, int64_t count, int dt, int vec, int *flag) {
+  const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  bool bad = false;
+  if (dt == IFM_F32 && !vec) {
+    for (int64_t i = tid; i < count; i += stride)
+      bad |= (((const uint32_t *)p)[i] & 0x7f800000u) == 0x7f800000u;
+  } else if (dt == IFM_F32) {  // 16-byte aligned: four values per load
+    const uint4 *q = (const uint4 *)p;
+    const int64_t n4 = count / 4;
+    for (int64_t i = tid; i < n4; i += stride) {
+      const uint4 v = q[i];
+      bad |= ((v.x & 0x7f800000u) == 0x7f800000u) | ((v.y & 0x7f800000u) == 0x7f800000u) |
+             ((v.z & 0x7f800000u) == 0x7f800000u) | ((v.w & 0x7f800000u) == 0x7f800000u);
+    }
+    for (int64_t i = 4 * n4 + tid; i < count; i += stride)
+      bad |= (((const uint32_t *)p)[i] & 0x7f800000u) == 0x7f800000u;
+  } else if (dt == IFM_F64) {
+    for (int64_t i = tid; i < count; i += stride)
+      bad |= (((const uint64_t *)p)[i] & 0x7ff0000000000000ull) == 0x7ff0000000000000ull;
+  } else {  // BF16 / F16
+    const uint32_t m = dt == IFM_BF16 ? 0x7f80u : 0x7c00u;
+    for (int64_t i = tid; i < count; i += stride) bad |= (((const uint16_t *)p)[i] & m) == m;
+  }
+  if (bad) atomicOr(flag, 1);
+}
+
 // ---------------------------------------------------------------------------
 extern "C" {
 
@@ -274,6 +303,37 @@ int ifm_ctx_create(int device, void *stream, ifm_ctx **out) {
   return IFM_OK;
 }
 
+int ifm_all_finite(ifm_ctx *ctx, const ifm_array *a, int *result) {
+  if (!ctx) return IFM_E_ARG;
+  int rc;
+  if ((rc = check_array(ctx, a, "a"))) return rc;
+  if (!result) return fail(ctx, IFM_E_ARG, "result is NULL");
+  if (!a->device) return fail(ctx, IFM_E_ARG, "ifm_all_finite takes a device array");
+  *result = 1;
+  const int dt = a->dtype;
+  if (!(dt == IFM_F32 || dt == IFM_F64 || dt == IFM_BF16 || dt == IFM_F16)) return IFM_OK;  // integers and N0f8
+  Geo g;
+  if (!layout_of(*a, g)) return fail(ctx, IFM_E_UNSUPPORTED, "a is not a dense array");
+  const int64_t count = count_of(*a);
+  if (count == 0) return IFM_OK;
+  if (!ctx->finite_flag && hipMalloc(&ctx->finite_flag, sizeof(int)) != hipSuccess) {
+    ctx->finite_flag = nullptr;
+    return fail(ctx, IFM_E_NOMEM, "device allocation failed (finite flag)");
+  }
+  HIPCHK(hipMemsetAsync(ctx->finite_flag, 0, sizeof(int), ctx->stream));
+  const bool vec = dt == IFM_F32 && ((uintptr_t)a->data & 15) == 0;
+  const int64_t units = vec ? (count + 3) / 4 : count;
+  const int64_t blocks = std::min<int64_t>((units + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_all_finite, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, a->data, count, dt,
+                     vec ? 1 : 0, ctx->finite_flag);
+  HIPCHK(hipGetLastError());
+  int h = 0;
+  HIPCHK(hipMemcpyAsync(&h, ctx->finite_flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  *result = h ? 0 : 1;
+  return IFM_OK;
+}
+
 int ifm_ctx_destroy(ifm_ctx *ctx) {
   if (!ctx) return IFM_OK;
   (void)hipStreamSynchronize(ctx->stream);
@@ -293,6 +353,7 @@ int ifm_ctx_destroy(ifm_ctx *ctx) {
   for (auto &e : ctx->tile_spec) (void)hipFree(e.spec);
   if (ctx->tile_taps) (void)hipFree(ctx->tile_taps);
   if (ctx->tile_flag) (void)hipFree(ctx->tile_flag);
+  if (ctx->finite_flag) (void)hipFree(ctx->finite_flag);
   batch_release(ctx);
   delete ctx;
   return IFM_OK;
@@ -1020,7 +1081,9 @@ int ifm_mapwindow_median(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, cons
   int rc;
   if ((rc = common_setup(ctx, in, out, g))) return rc;
   if ((rc = check_border(ctx, b))) return rc;
-  if (in->dtype != out->dtype) return fail(ctx, IFM_E_ARG, "median output holds the input eltype");
+  const bool codes_in = in->dtype == IFM_U8 || in->dtype == IFM_N0F8;
+  if (in->dtype != out->dtype && !(codes_in && (out->dtype == IFM_F32 || out->dtype == IFM_F64)))
+    return fail(ctx, IFM_E_ARG, "median output holds the input eltype (or Float32 / Float64 for UInt8 / N0f8)");
   int wf[MAXD], wl[MAXD];
   if ((rc = window_setup(ctx, g, win_first, win_len, wf, wl))) return rc;
   int64_t L = (int64_t)wl[0] * wl[1] * wl[2] * wl[3];

@@ -40,6 +40,15 @@ __host__ __device__ inline int bmap(int style, int i, int n) {
   }
 }
 
+// N0f8 code / 255 rounded to nearest (Float32(x::N0f8)): the product with the
+// rounded reciprocal corrected by one FMA step, which equals the correctly
+// rounded quotient for every code 0..255 (checked exhaustively on the CPU).
+__host__ __device__ __forceinline__ float n0f8_to_f32(uint32_t code) {
+  const float x = (float)code, r = 1.0f / 255.0f;
+  const float q = x * r;
+  return fmaf(fmaf(-q, 255.0f, x), r, q);
+}
+
 // Element access with a runtime eltype: the branch is uniform over the grid.
 __device__ __forceinline__ float bf16_to_f32(uint16_t b) { return __uint_as_float(((uint32_t)b) << 16); }
 __device__ __forceinline__ uint16_t f32_to_bf16(float f) {
@@ -52,7 +61,7 @@ __device__ __forceinline__ uint16_t f32_to_bf16(float f) {
 __device__ __forceinline__ float load_f(const void *p, int64_t i, int dt) {
   switch (dt) {
   case IFM_U8: return (float)((const uint8_t *)p)[i];
-  case IFM_N0F8: return (float)((const uint8_t *)p)[i] * (1.0f / 255.0f);
+  case IFM_N0F8: return n0f8_to_f32(((const uint8_t *)p)[i]);
   case IFM_F32: return ((const float *)p)[i];
   case IFM_F64: return (float)((const double *)p)[i];
   case IFM_BF16: return bf16_to_f32(((const uint16_t *)p)[i]);
@@ -127,6 +136,19 @@ __device__ __forceinline__ void store_key32(void *p, int64_t i, int dt, uint32_t
     ((uint16_t *)p)[i] = (uint16_t)u;
     break;
   }
+  }
+}
+
+// An order-statistic key stored in the output eltype odt: the input eltype
+// itself, or for UInt8 / N0f8 codes the Float32 / Float64 value (the eltype of
+// Statistics.middle of such an element).
+__device__ __forceinline__ void store_key_as(void *p, int64_t i, int dt, int odt, uint32_t k) {
+  if (odt == dt) {
+    store_key32(p, i, dt, k);
+  } else if (odt == IFM_F32) {
+    ((float *)p)[i] = dt == IFM_N0F8 ? n0f8_to_f32(k) : (float)k;
+  } else {
+    ((double *)p)[i] = dt == IFM_N0F8 ? (double)k / 255.0 : (double)k;
   }
 }
 
@@ -252,6 +274,7 @@ struct MedianArgs {
   int nplanes, o2, n2, f2, f3;
   int style;
   int dt;
+  int odt;  // output eltype: dt, or F32 / F64 for UInt8 / N0f8 codes
   uint32_t fillkey;
 };
 hipError_t launch_median(const Dense4 &in, Dense4 &out, const int64_t *out_first, const int *wfirst,
@@ -266,7 +289,7 @@ struct BandJob {  // one band strip (out indices) of a circular median
   int n0, n1, nplanes, opitch;
   int f0, f1;
   int x0, y0, w, h;
-  int dt;
+  int dt, odt;
 };
 int median_band_jobs(const Dense4 &in, const Dense4 &out, const int64_t *out_first, int k, BandJob *jobs);
 hipError_t launch_median_band_batch(int k, const BandJob *jobs, const int *tile_req, const int *tile_first, int ntiles,
@@ -391,6 +414,7 @@ struct ifm_ctx {
   int tile_gen = 0;
   // device address of the no-alloc scratch array on this ctx's device
   void *noalloc_scratch = nullptr;
+  int *finite_flag = nullptr;  // ifm_all_finite's result word
 };
 
 namespace ifm {

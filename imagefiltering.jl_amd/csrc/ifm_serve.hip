@@ -351,6 +351,7 @@ int build_batch(ifm_ctx *ctx, const ifm_request *reqs, const int *idx, int n, co
         a.nplanes = a.o2 = a.n2 = 1;
         a.style = q.border.style;
         a.dt = vin[i].dtype;
+        a.odt = vin[i].dtype;
         a.fillkey = median_fill_key(vin[i].dtype, q.border.fill);
         int bx, by;
         median_batch_tile(k, pack == 1, &bx, &by);

@@ -212,7 +212,7 @@ __device__ __forceinline__ void mf_values(const MfmaArgs &a, const MfTile &T, ui
     break;
   case IFM_N0F8:
 #pragma unroll
-    for (int j = 0; j < G::JB; ++j) raw[j] = __float_as_uint((float)raw[j] * (1.0f / 255.0f));
+    for (int j = 0; j < G::JB; ++j) raw[j] = __float_as_uint(n0f8_to_f32(raw[j]));
     break;
   case IFM_BF16:
 #pragma unroll

@@ -153,8 +153,9 @@ __device__ __forceinline__ void median_tile(const MedianArgs &a, int bx, int by,
         int ox = bx + cx + c;
         int oy = by + cy + r;
         if (ox < a.o0) {
-          if (oy < a.o1) store_key32(a.out, obase + (int64_t)oy * a.o0 + ox, a.dt, o[c + TX * r].x);
-          if (oy + BYH < a.o1) store_key32(a.out, obase + (int64_t)(oy + BYH) * a.o0 + ox, a.dt, o[c + TX * r].y);
+          if (oy < a.o1) store_key_as(a.out, obase + (int64_t)oy * a.o0 + ox, a.dt, a.odt, o[c + TX * r].x);
+          if (oy + BYH < a.o1)
+            store_key_as(a.out, obase + (int64_t)(oy + BYH) * a.o0 + ox, a.dt, a.odt, o[c + TX * r].y);
         }
       }
   } else {
@@ -171,7 +172,7 @@ __device__ __forceinline__ void median_tile(const MedianArgs &a, int bx, int by,
       for (int c = 0; c < TX; ++c) {
         int ox = bx + cx + c;
         int oy = by + cy + r;
-        if (ox < a.o0 && oy < a.o1) store_key32(a.out, obase + (int64_t)oy * a.o0 + ox, a.dt, o[c + TX * r]);
+        if (ox < a.o0 && oy < a.o1) store_key_as(a.out, obase + (int64_t)oy * a.o0 + ox, a.dt, a.odt, o[c + TX * r]);
       }
   }
 }
@@ -216,7 +217,7 @@ struct MedianAnyArgs {
   int wf[MAXD];
   int wl[MAXD];
   int style;
-  int dt;
+  int dt, odt;
   double fill;
   int64_t total;
   int L;
@@ -297,7 +298,7 @@ __global__ void __launch_bounds__(256) k_median_any(MedianAnyArgs a) {
       }
     }
     if constexpr (sizeof(KT) == 8) store_key64(a.out, oidx, result);
-    else store_key32(a.out, oidx, a.dt, result);
+    else store_key_as(a.out, oidx, a.dt, a.odt, result);
   }
 }
 
@@ -399,6 +400,7 @@ static hipError_t median_any_region(const Dense4 &in, Dense4 &out, const int64_t
   if (a.total == 0) return hipSuccess;
   a.style = style;
   a.dt = in.dtype;
+  a.odt = out.dtype;
   a.fill = fill;
   int64_t blocks = (a.total + 255) / 256;
   if (blocks > 1 << 20) blocks = 1 << 20;
@@ -449,7 +451,7 @@ __device__ __forceinline__ void band_one(const BandJob &j, int local, int plane)
     }
     res = (less <= rank && rank < less + eq) ? k[i] : res;
   }
-  store_key32(j.out, (int64_t)plane * j.out_plane + (int64_t)oy * j.opitch + ox, j.dt, res);
+  store_key_as(j.out, (int64_t)plane * j.out_plane + (int64_t)oy * j.opitch + ox, j.dt, j.odt, res);
 }
 
 struct BandJobs4 {
@@ -501,6 +503,7 @@ int median_band_jobs(const Dense4 &in, const Dense4 &out, const int64_t *out_fir
     j.w = (int)rect[s][2];
     j.h = (int)rect[s][3];
     j.dt = in.dtype;
+    j.odt = out.dtype;
   }
   return n;
 }
@@ -564,6 +567,7 @@ hipError_t launch_median(const Dense4 &in, Dense4 &out, const int64_t *out_first
     a.f3 = (int)out_first[3];
     a.style = style;
     a.dt = in.dtype;
+    a.odt = out.dtype;
     a.fillkey = fill_key32(in.dtype, fill);
     bool pack = in.dtype != IFM_F32;
     if (k == 3) {

@@ -68,7 +68,7 @@ __device__ __forceinline__ float raw_val(uint32_t r) {
   else if constexpr (DT == IFM_BF16) return __uint_as_float(r << 16);
   else if constexpr (DT == IFM_F16) return (float)__builtin_bit_cast(_Float16, (uint16_t)r);
   else if constexpr (DT == IFM_U8) return (float)r;
-  else return (float)r * (1.0f / 255.0f);
+  else return n0f8_to_f32(r);
 }
 
 // Thread's staging coordinates: column c (of in_w), rows rh, rh + 2, ...

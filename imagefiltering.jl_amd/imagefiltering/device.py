@@ -12,6 +12,40 @@ from . import _lib
 from .eltypes import BF16Array, BFloat16, N0f8, N0f8Array, eltype_of
 
 
+# Caching allocator: freed device blocks are kept by rounded size and handed to
+# the next array of that size, so repeated calls (imfilter allocates its
+# output) do not pay a hipMalloc / hipFree each.  Every library call runs on
+# the one context stream, so a block reused while earlier work on it is still
+# queued is ordered after that work.
+_POOL = {}
+_POOL_BYTES = [0]
+POOL_LIMIT = 16 << 30
+
+
+def _rounded(nbytes):
+    nbytes = max(int(nbytes), 16)
+    gran = 2 << 20 if nbytes >= (1 << 20) else 4096
+    return (nbytes + gran - 1) // gran * gran
+
+
+def _pool_take(size):
+    blocks = _POOL.get(size)
+    if blocks:
+        _POOL_BYTES[0] -= size
+        return blocks.pop()
+    return None
+
+
+def empty_cache():
+    """Frees every cached block (torch.cuda.empty_cache's role)."""
+    L = _lib.load()
+    for size, blocks in list(_POOL.items()):
+        for p in blocks:
+            L.ifm_device_free(_lib.ctx(), C.c_void_p(p))
+    _POOL.clear()
+    _POOL_BYTES[0] = 0
+
+
 class DeviceArray:
     def __init__(self, shape, eltype, order="F"):
         self.shape = tuple(int(s) for s in shape)
@@ -19,8 +53,18 @@ class DeviceArray:
         self.order = order
         n = int(np.prod(self.shape)) if self.shape else 1
         self.nbytes = n * eltype.itemsize
+        self._size = _rounded(self.nbytes)
         self._ptr = C.c_void_p()
-        _lib.check(_lib.load().ifm_device_alloc(_lib.ctx(), max(self.nbytes, 16), C.byref(self._ptr)))
+        p = _pool_take(self._size)
+        if p is not None:
+            self._ptr = C.c_void_p(p)
+            return
+        L = _lib.load()
+        rc = L.ifm_device_alloc(_lib.ctx(), self._size, C.byref(self._ptr))
+        if rc == _lib.E_NOMEM and _POOL_BYTES[0]:
+            empty_cache()
+            rc = L.ifm_device_alloc(_lib.ctx(), self._size, C.byref(self._ptr))
+        _lib.check(rc)
 
     @property
     def ptr(self):
@@ -66,7 +110,11 @@ class DeviceArray:
     def __del__(self):
         try:
             if self._ptr and self._ptr.value and _lib._lib is not None and _lib._ctx is not None:
-                _lib._lib.ifm_device_free(_lib._ctx, self._ptr)
+                if _POOL_BYTES[0] + self._size <= POOL_LIMIT:
+                    _POOL.setdefault(self._size, []).append(self._ptr.value)
+                    _POOL_BYTES[0] += self._size
+                else:
+                    _lib._lib.ifm_device_free(_lib._ctx, self._ptr)
                 self._ptr = C.c_void_p()
         except Exception:
             pass

@@ -245,9 +245,11 @@ def _all_finite(img):
 def _device_all_finite(img):
     if img.eltype.kind != "float":
         return True
-    # the scan stays on the device: a min/max over the whole array propagates NaN/Inf
-    from .mapwindow import _finite_scan
-    return _finite_scan(img)
+    # one streaming read on the device; only the answer crosses to the host
+    import ctypes
+    r = ctypes.c_int(0)
+    _lib.check(_lib.load().ifm_all_finite(_lib.ctx(), ifm_array(img), ctypes.byref(r)))
+    return bool(r.value)
 
 
 def filter_algorithm(img, kernel):

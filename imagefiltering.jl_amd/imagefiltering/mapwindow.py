@@ -112,15 +112,6 @@ def _region(img, win, border, indices):
     return [0] * N, list(img.shape)
 
 
-def _convert_device(src, T):
-    """Element-type conversion on the device (ifm_padarray with zero padding)."""
-    dst = empty_like(src, T)
-    b = _lib.IfmBorder()
-    b.style = _lib.REPLICATE
-    _lib.check(_lib.load().ifm_padarray(_lib.ctx(), ifm_array(src), ifm_array(dst), b))
-    return dst
-
-
 def mapwindow(f, img, window, border=None, indices=None, callmode="copy!"):
     """mapwindow(f, img, window; border="replicate", indices=axes(img)) (src/mapwindow.jl:75-85)."""
     if callmode != "copy!":
@@ -153,10 +144,11 @@ def mapwindow(f, img, window, border=None, indices=None, callmode="copy!"):
         Lw = int(np.prod([len(w) for w in win]))
         if f is median and Lw % 2 == 0:
             raise NotImplementedError("Statistics.median over an even window (mean of two ranks)")
-        codes = empty_like(img, S, shape)
-        _lib.check(L.ifm_mapwindow_median(ctx, ifm_array(img), ifm_array(codes), of, wf, wl, b))
+        # the kernel writes typeof(middle(x)) directly (Float32 for N0f8,
+        # Float64 for UInt8, the eltype itself for floats)
         T = _median_eltype(S)
-        res = codes if T is S else _convert_device(codes, T)
+        res = empty_like(img, T, shape)
+        _lib.check(L.ifm_mapwindow_median(ctx, ifm_array(img), ifm_array(res), of, wf, wl, b))
         return _wrap(res, first, border, indices)
     if f in (minimum, maximum, extrema):
         omin = empty_like(img, S, shape) if f in (minimum, extrema) else None
@@ -217,19 +209,3 @@ def mapwindow_(f, out, img, window, border=None, indices=None):
         raise DimensionMismatch("out has the wrong size")
     np.copyto(np.asarray(target), np.asarray(res), casting="unsafe")
     return out
-
-
-def _finite_scan(img):
-    """all(isfinite, img) for a DeviceArray, on the device: isfinite fails
-    exactly where the maximum of |x| over the array is Inf or NaN."""
-    mx = empty_like(img, img.eltype, (1,) * img.ndim)
-    mn = empty_like(img, img.eltype, (1,) * img.ndim)
-    wf = _lib.i64([0] * img.ndim)
-    wl = _lib.i64(list(img.shape), 1)
-    b = _lib.IfmBorder()
-    b.style = _lib.REPLICATE
-    _lib.check(_lib.load().ifm_mapwindow_minmax(_lib.ctx(), ifm_array(img), ifm_array(mn), ifm_array(mx),
-                                                _lib.i64([0] * img.ndim), wf, wl, b))
-    v = np.concatenate([np.asarray(mn.to_host(), dtype=np.float64).ravel(),
-                        np.asarray(mx.to_host(), dtype=np.float64).ravel()])
-    return bool(np.isfinite(v).all())

@@ -157,6 +157,12 @@ int ifm_imfilter_fir_noalloc(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, 
 int ifm_imfilter_dense(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const int64_t *out_first,
                        const ifm_kernel *kernel, const ifm_border *border, int precision);
 
+/* all(isfinite, a) for a device array (*result = 1 if every value is finite;
+ * integer and N0f8 arrays are always finite): the scan filter_algorithm makes
+ * before it picks FFT for kernels longer than 30 (src/imfilter.jl:1200-1209).
+ * One streaming read of the array; returns after the answer is on the host. */
+int ifm_all_finite(ifm_ctx *ctx, const ifm_array *a, int *result);
+
 /* ---- imfilter: FFT ------------------------------------------------------- */
 /* Replaces _imfilter_fft! / filtfft (src/imfilter.jl:796-866): the
  * correlation of the border-extended image with the kernel, computed by FFT.
@@ -184,7 +190,8 @@ int ifm_imfilter_iir(ifm_ctx *ctx, const ifm_array *in, ifm_array *out,
 /* Replaces mapwindow(median!, img, window; border) -> median_fast!
  * (src/mapwindow.jl:123-131, :270-333): the ((L+1)÷2)-th smallest of the
  * L window values under isless order.  win_first[d]/win_len[d] give the window
- * offsets.  out holds the selected element in the input eltype. */
+ * offsets.  out holds the selected element in the input eltype, or for UInt8 /
+ * N0f8 input its Float32 / Float64 value (typeof(middle(x)), written directly). */
 int ifm_mapwindow_median(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const int64_t *out_first,
                          const int64_t *win_first, const int64_t *win_len, const ifm_border *border);
 

@@ -24,8 +24,33 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define MAXD 4
+
+/* Threads of the loops below (the all-core "parallel restatement" of the CPU
+ * baseline).  Every output element is computed by the same operations in the
+ * same order whatever the thread count, so results do not depend on it.
+ * Without OpenMP (the default build used by the tests) this is one thread. */
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+  omp_set_num_threads(n > 0 ? n : 1);
+  return n > 0 ? n : 1;
+#else
+  (void)n;
+  return 1;
+#endif
+}
+
+int orc_max_threads(void) {
+#ifdef _OPENMP
+  return omp_get_num_procs();
+#else
+  return 1;
+#endif
+}
 
 static int64_t jmod(int64_t x, int64_t m) { /* Julia mod for m > 0 */
   int64_t r = x % m;
@@ -133,20 +158,23 @@ int orc_imfilter_dense(const double *img, int ndim, const int64_t *n, const doub
     cnt *= P[d];
   }
   double *pad = (double *)malloc(sizeof(double) * (size_t)(cnt ? cnt : 1));
-  int64_t x[MAXD], p = 0;
-  for (int64_t i3 = 0; i3 < P[3]; ++i3)
-    for (int64_t i2 = 0; i2 < P[2]; ++i2)
-      for (int64_t i1 = 0; i1 < P[1]; ++i1)
-        for (int64_t i0 = 0; i0 < P[0]; ++i0, ++p) {
-          x[0] = PO[0] + i0; x[1] = PO[1] + i1; x[2] = PO[2] + i2; x[3] = PO[3] + i3;
-          pad[p] = padval(img, N, style, fill, x);
-        }
+  const int64_t prow = P[1] * P[2] * P[3];
+#pragma omp parallel for schedule(static)
+  for (int64_t r = 0; r < prow; ++r) {
+    int64_t x[MAXD];
+    const int64_t i1 = r % P[1], i2 = (r / P[1]) % P[2], i3 = r / (P[1] * P[2]);
+    for (int64_t i0 = 0; i0 < P[0]; ++i0) {
+      x[0] = PO[0] + i0; x[1] = PO[1] + i1; x[2] = PO[2] + i2; x[3] = PO[3] + i3;
+      pad[i0 + P[0] * r] = padval(img, N, style, fill, x);
+    }
+  }
   /* __imfilter_inbounds!: tmp += A[i+j, I+J] * k[j, J], j (dim 1) innermost */
-  p = 0;
-  for (int64_t o3 = 0; o3 < OS[3]; ++o3)
-    for (int64_t o2 = 0; o2 < OS[2]; ++o2)
-      for (int64_t o1 = 0; o1 < OS[1]; ++o1)
-        for (int64_t o0 = 0; o0 < OS[0]; ++o0, ++p) {
+  const int64_t orow = OS[1] * OS[2] * OS[3];
+#pragma omp parallel for schedule(static)
+  for (int64_t r = 0; r < orow; ++r) {
+    const int64_t o1 = r % OS[1], o2 = (r / OS[1]) % OS[2], o3 = r / (OS[1] * OS[2]);
+    for (int64_t o0 = 0; o0 < OS[0]; ++o0) {
+          const int64_t p = o0 + OS[0] * r;
           double tmp = 0.0;
           const double *kq = k;
           for (int64_t j3 = 0; j3 < KS[3]; ++j3)
@@ -157,7 +185,8 @@ int orc_imfilter_dense(const double *img, int ndim, const int64_t *n, const doub
                 kq += KS[0];
               }
           out[p] = tmp;
-        }
+    }
+  }
   free(pad);
   return 0;
 }
@@ -218,11 +247,13 @@ int orc_imfilter_factors(const double *img, int ndim, const int64_t *n, int nf, 
     int64_t ext[MAXD], cnt = 1;
     for (int d = 0; d < MAXD; ++d) { ext[d] = rhi[s][d] - rlo[s][d] + 1; cnt *= ext[d]; }
     double *cur = (s == nk - 1) ? out : (double *)malloc(sizeof(double) * (size_t)(cnt ? cnt : 1));
-    int64_t p = 0, x[MAXD];
-    for (int64_t i3 = 0; i3 < ext[3]; ++i3)
-      for (int64_t i2 = 0; i2 < ext[2]; ++i2)
-        for (int64_t i1 = 0; i1 < ext[1]; ++i1)
-          for (int64_t i0 = 0; i0 < ext[0]; ++i0, ++p) {
+    const int64_t nrow = ext[1] * ext[2] * ext[3];
+#pragma omp parallel for schedule(static)
+    for (int64_t r = 0; r < nrow; ++r) {
+      int64_t x[MAXD];
+      const int64_t i1 = r % ext[1], i2 = (r / ext[1]) % ext[2], i3 = r / (ext[1] * ext[2]);
+          for (int64_t i0 = 0; i0 < ext[0]; ++i0) {
+            const int64_t p = i0 + ext[0] * r;
             double tmp = 0.0;
             for (int j = 0; j < flen[f]; ++j) {
               x[0] = rlo[s][0] + i0; x[1] = rlo[s][1] + i1; x[2] = rlo[s][2] + i2; x[3] = rlo[s][3] + i3;
@@ -239,6 +270,7 @@ int orc_imfilter_factors(const double *img, int ndim, const int64_t *n, int nf, 
             }
             cur[p] = tmp;
           }
+    }
     if (prev) free(prev);
     prev = (s == nk - 1) ? NULL : cur;
     for (int d = 0; d < MAXD; ++d) { plo[d] = rlo[s][d]; pext[d] = ext[d]; }
@@ -260,6 +292,7 @@ static void iir_axis(const double *src, double *dst, const int64_t *N, int axis,
   int64_t n = N[axis], stride = 1;
   for (int d = 0; d < axis; ++d) stride *= N[d];
   int64_t total = N[0] * N[1] * N[2] * N[3], lines = total / n;
+#pragma omp parallel for schedule(static)
   for (int64_t L = 0; L < lines; ++L) {
     int64_t pre = L % stride, post = L / stride;
     int64_t base = pre + post * stride * n;
@@ -352,10 +385,9 @@ static int cmpkey(const void *x, const void *y) {
  * the image are copied; at the edge, copy_win! maps every axis with
  * padindex(border, inner, outer) where inner is the window's intersection
  * with the image axes (:310-316), or fills with the Fill value (:323-331). */
-static int g_window_err; /* set when an edge window has an empty or zero-period index map */
-
+/* *err is set when an edge window has an empty or zero-period index map. */
 static int64_t gather_window(const double *img, const int64_t *N, int style, double fill,
-                             const int64_t *q, const int64_t *WF, const int64_t *WL, double *buf) {
+                             const int64_t *q, const int64_t *WF, const int64_t *WL, double *buf, int *err) {
   int64_t ilo[MAXD], ilen[MAXD];
   for (int d = 0; d < MAXD; ++d) {
     int64_t a = q[d] + WF[d], b = q[d] + WF[d] + WL[d] - 1;
@@ -374,7 +406,7 @@ static int64_t gather_window(const double *img, const int64_t *N, int style, dou
             if (x[d] >= 0 && x[d] < N[d]) { s[d] = x[d]; continue; }
             outside = 1;
             if (style == 4) { s[d] = -1; continue; }
-            if (ilen[d] <= 0 || (style == 3 && ilen[d] == 1)) { g_window_err = 1; s[d] = -1; continue; }
+            if (ilen[d] <= 0 || (style == 3 && ilen[d] == 1)) { *err = 1; s[d] = -1; continue; }
             s[d] = ilo[d] + orc_padmap(style, x[d] - ilo[d], ilen[d]);
           }
           int isfill = 0;
@@ -401,16 +433,20 @@ int orc_median_f64(const double *img, int ndim, const int64_t *n, const int64_t 
     OS[d] = d < ndim ? (osize ? osize[d] : N[d]) : 1;
     L *= WL[d];
   }
+  int err = 0;
+  const int64_t orow = OS[1] * OS[2] * OS[3];
+#pragma omp parallel reduction(| : err)
+  {
   double *buf = (double *)malloc(sizeof(double) * (size_t)L);
   uint64_t *key = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)L);
-  g_window_err = 0;
-  int64_t p = 0, q[MAXD];
-  for (int64_t o3 = 0; o3 < OS[3]; ++o3)
-    for (int64_t o2 = 0; o2 < OS[2]; ++o2)
-      for (int64_t o1 = 0; o1 < OS[1]; ++o1)
-        for (int64_t o0 = 0; o0 < OS[0]; ++o0, ++p) {
+  int64_t q[MAXD];
+#pragma omp for schedule(static)
+  for (int64_t r = 0; r < orow; ++r) {
+    const int64_t o1 = r % OS[1], o2 = (r / OS[1]) % OS[2], o3 = r / (OS[1] * OS[2]);
+        for (int64_t o0 = 0; o0 < OS[0]; ++o0) {
+          const int64_t p = o0 + OS[0] * r;
           q[0] = OF[0] + o0; q[1] = OF[1] + o1; q[2] = OF[2] + o2; q[3] = OF[3] + o3;
-          int64_t m = gather_window(img, N, style, fill, q, WF, WL, buf);
+          int64_t m = gather_window(img, N, style, fill, q, WF, WL, buf, &err);
           for (int64_t t = 0; t < m; ++t) key[t] = fkey(buf[t]);
           qsort(key, (size_t)m, sizeof(uint64_t), cmpkey);
           uint64_t kk = key[(m + 1) / 2 - 1];
@@ -423,8 +459,10 @@ int orc_median_f64(const double *img, int ndim, const int64_t *n, const int64_t 
           }
           out[p] = v;
         }
+  }
   free(buf); free(key);
-  return g_window_err ? -1 : 0;
+  }
+  return err ? -1 : 0;
 }
 
 /* Julia min/max on floats: NaN-propagating, -0.0 < +0.0. */
@@ -454,22 +492,28 @@ int orc_minmax_f64(const double *img, int ndim, const int64_t *n, const int64_t 
     OS[d] = d < ndim ? (osize ? osize[d] : N[d]) : 1;
     L *= WL[d];
   }
+  int err = 0;
+  const int64_t orow = OS[1] * OS[2] * OS[3];
+#pragma omp parallel reduction(| : err)
+  {
   double *buf = (double *)malloc(sizeof(double) * (size_t)L);
-  g_window_err = 0;
-  int64_t p = 0, q[MAXD];
-  for (int64_t o3 = 0; o3 < OS[3]; ++o3)
-    for (int64_t o2 = 0; o2 < OS[2]; ++o2)
-      for (int64_t o1 = 0; o1 < OS[1]; ++o1)
-        for (int64_t o0 = 0; o0 < OS[0]; ++o0, ++p) {
+  int64_t q[MAXD];
+#pragma omp for schedule(static)
+  for (int64_t r = 0; r < orow; ++r) {
+    const int64_t o1 = r % OS[1], o2 = (r / OS[1]) % OS[2], o3 = r / (OS[1] * OS[2]);
+        for (int64_t o0 = 0; o0 < OS[0]; ++o0) {
+          const int64_t p = o0 + OS[0] * r;
           q[0] = OF[0] + o0; q[1] = OF[1] + o1; q[2] = OF[2] + o2; q[3] = OF[3] + o3;
-          int64_t m = gather_window(img, N, style, fill, q, WF, WL, buf);
+          int64_t m = gather_window(img, N, style, fill, q, WF, WL, buf, &err);
           double mn = buf[0], mx = buf[0];
           for (int64_t t = 1; t < m; ++t) { mn = jmin(mn, buf[t]); mx = jmax(mx, buf[t]); }
           if (omin) omin[p] = mn;
           if (omax) omax[p] = mx;
         }
+  }
   free(buf);
-  return g_window_err ? -1 : 0;
+  }
+  return err ? -1 : 0;
 }
 
 /* Lemire streaming min/max along axis 0 of (mn, mx) pairs
@@ -481,12 +525,15 @@ static void lemire_axis(double *mn, double *mx, const int64_t *N, int axis, int6
   int64_t total = N[0] * N[1] * N[2] * N[3], lines = total / n;
   int64_t h = w >> 1;
   int64_t cap = w + 1;
+  /* cache seed: first(A) of the whole array (:390-393) */
+  const double a0n = mn[0], a0x = mx[0];
+#pragma omp parallel
+  {
   int64_t *L = (int64_t *)malloc(sizeof(int64_t) * (size_t)cap);
   int64_t *U = (int64_t *)malloc(sizeof(int64_t) * (size_t)cap);
   double *cmn = (double *)malloc(sizeof(double) * (size_t)(h + 1));
   double *cmx = (double *)malloc(sizeof(double) * (size_t)(h + 1));
-  /* cache seed: first(A) of the whole array (:390-393) */
-  double a0n = mn[0], a0x = mx[0];
+#pragma omp for schedule(static)
   for (int64_t ln = 0; ln < lines; ++ln) {
     int64_t pre = ln % stride, post = ln / stride;
     int64_t base = pre + post * stride * n;
@@ -533,6 +580,7 @@ static void lemire_axis(double *mn, double *mx, const int64_t *N, int axis, int6
 #undef MX
   }
   free(L); free(U); free(cmn); free(cmx);
+  }
 }
 
 /* mapwindow(extrema, A, window::Dims) -> extrema_filter (src/mapwindow.jl:

@@ -17,33 +17,58 @@ _LIB = None
 STYLES = {"replicate": 0, "circular": 1, "symmetric": 2, "reflect": 3, "fill": 4}
 
 
+NATIVE_PATH = os.path.join(_HERE, "_native", "liboracle.so")
+
+
 def lib():
     global _LIB
     if _LIB is None:
         path = os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
-        L = C.CDLL(path)
-        i64p = C.POINTER(C.c_int64)
-        dp = C.POINTER(C.c_double)
-        ip = C.POINTER(C.c_int)
-        L.orc_padmap.restype = C.c_int64
-        L.orc_padmap.argtypes = [C.c_int, C.c_int64, C.c_int64]
-        L.orc_padindex.argtypes = [C.c_int, C.c_int64, C.c_int64, C.c_int64, i64p]
-        L.orc_padarray_f64.argtypes = [dp, C.c_int, i64p, C.c_int, C.c_double, i64p, i64p, dp]
-        L.orc_imfilter_dense.argtypes = [dp, C.c_int, i64p, dp, i64p, i64p, C.c_int, C.c_double,
-                                         i64p, i64p, dp]
-        L.orc_imfilter_factors.argtypes = [dp, C.c_int, i64p, C.c_int, ip, ip, i64p, dp, C.c_int,
-                                           C.c_double, i64p, i64p, dp]
-        L.orc_imfilter_iir.argtypes = [dp, C.c_int, i64p, C.c_int, dp, ip, C.c_int, C.c_double, dp]
-        L.orc_median_f64.argtypes = [dp, C.c_int, i64p, i64p, i64p, C.c_int, C.c_double, i64p, i64p, dp]
-        L.orc_minmax_f64.argtypes = [dp, C.c_int, i64p, i64p, i64p, C.c_int, C.c_double, i64p, i64p,
-                                     dp, dp]
-        L.orc_extrema_lemire_f64.argtypes = [dp, C.c_int, i64p, i64p, dp, dp]
-        L.orc_imfilter_dense_rows.argtypes = [dp, C.c_int, i64p, dp, i64p, i64p, C.c_int, C.c_double,
-                                              C.c_int64, C.c_int64, dp]
-        _LIB = L
+        _LIB = _open(path)
     return _LIB
+
+
+def use_library(path):
+    """Switch to another build of the same source (the -O3 -march=native
+    OpenMP build the CPU baseline times); returns its thread count."""
+    global _LIB
+    _LIB = _open(path)
+    return _LIB.orc_max_threads()
+
+
+def set_threads(n):
+    return lib().orc_set_threads(int(n))
+
+
+def max_threads():
+    return lib().orc_max_threads()
+
+
+def _open(path):
+    L = C.CDLL(path)
+    i64p = C.POINTER(C.c_int64)
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int)
+    L.orc_padmap.restype = C.c_int64
+    L.orc_padmap.argtypes = [C.c_int, C.c_int64, C.c_int64]
+    L.orc_padindex.argtypes = [C.c_int, C.c_int64, C.c_int64, C.c_int64, i64p]
+    L.orc_padarray_f64.argtypes = [dp, C.c_int, i64p, C.c_int, C.c_double, i64p, i64p, dp]
+    L.orc_imfilter_dense.argtypes = [dp, C.c_int, i64p, dp, i64p, i64p, C.c_int, C.c_double,
+                                     i64p, i64p, dp]
+    L.orc_imfilter_factors.argtypes = [dp, C.c_int, i64p, C.c_int, ip, ip, i64p, dp, C.c_int,
+                                       C.c_double, i64p, i64p, dp]
+    L.orc_imfilter_iir.argtypes = [dp, C.c_int, i64p, C.c_int, dp, ip, C.c_int, C.c_double, dp]
+    L.orc_median_f64.argtypes = [dp, C.c_int, i64p, i64p, i64p, C.c_int, C.c_double, i64p, i64p, dp]
+    L.orc_minmax_f64.argtypes = [dp, C.c_int, i64p, i64p, i64p, C.c_int, C.c_double, i64p, i64p,
+                                 dp, dp]
+    L.orc_extrema_lemire_f64.argtypes = [dp, C.c_int, i64p, i64p, dp, dp]
+    L.orc_imfilter_dense_rows.argtypes = [dp, C.c_int, i64p, dp, i64p, i64p, C.c_int, C.c_double,
+                                          C.c_int64, C.c_int64, dp]
+    L.orc_set_threads.argtypes = [C.c_int]
+    L.orc_max_threads.argtypes = []
+    return L
 
 
 def _i64(v):
@@ -190,3 +215,46 @@ def imfilter_dense_rows(img, k, kfirst, border, row0, nrows):
     lib().orc_imfilter_dense_rows(ap, a.ndim, _i64(a.shape), kp, _i64(kk.shape), _i64(kfirst), s, fill,
                                   row0, nrows, op)
     return o
+
+
+def _nextprod23(n):
+    best = None
+    p2 = 1
+    while p2 < 2 * n + 2:
+        p = p2
+        while p < 2 * n + 2:
+            if p >= n and (best is None or p < best):
+                best = p
+            p *= 3
+        p2 *= 2
+    return best
+
+
+def imfilter_fft(img, k, kfirst, border):
+    """Algorithm.FFT() restated with numpy's FFT in FP64 (src/imfilter.jl:796-866,
+    src/border.jl:240-248): pad every axis with the border to
+    nextprod([2,3], n + K) (the padding shifted by the kernel's first tap),
+    wrap the kernel into an FFTView of the padded size, and read back
+    irfft(rfft(Apad) .* conj(rfft(kpad))) at the image axes."""
+    a = np.asarray(img, dtype=np.float64)
+    k = np.asarray(k, dtype=np.float64).reshape(np.shape(k) + (1,) * (a.ndim - np.ndim(k)))
+    kfirst = list(kfirst) + [0] * (a.ndim - len(kfirst))
+    lo, hi, N = [], [], []
+    for d in range(a.ndim):
+        n, K, f = a.shape[d], k.shape[d], kfirst[d]
+        # padded index u holds image index u - lo; output c reads
+        # u = c + lo + f .. c + lo + f + K - 1, which must stay below N
+        l = max(-f, 0)
+        Nd = _nextprod23(max(n + K, n + max(f, 0) + K - 1)) if (K > 1 or f != 0) else n
+        lo.append(l)
+        hi.append(Nd - n - l)
+        N.append(Nd)
+    pad = padarray(a, border, lo, hi)
+    kp = np.zeros(N)
+    for idx in np.ndindex(*k.shape):
+        u = tuple((idx[d] + kfirst[d]) % N[d] for d in range(a.ndim))
+        kp[u] += k[idx]
+    ax = list(range(a.ndim))
+    R = np.fft.irfftn(np.fft.rfftn(pad, axes=ax) * np.conj(np.fft.rfftn(kp, axes=ax)), s=N, axes=ax)
+    sl = tuple(slice(lo[d], lo[d] + a.shape[d]) for d in range(a.ndim))
+    return np.asfortranarray(R[sl])

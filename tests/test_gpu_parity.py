@@ -781,3 +781,34 @@ def test_device_resident_roundtrip():
     assert isinstance(out, IF.DeviceArray)
     want = O.imfilter_factors(img.astype(np.float64), factors_of(kern), "symmetric")
     assert relerr(out.to_host(), want) <= TOL
+
+
+# ---- filter_algorithm's isfinite scan on device arrays ----------------------------------
+@pytest.mark.parametrize("dtype", ["float32", "float64", "bfloat16", "float16", "uint8"])
+def test_all_finite_device_scan(dtype):
+    from imagefiltering.imfilter import _device_all_finite
+    rng = np.random.default_rng(81)
+    base = rng.random(10007)   # odd length: a tail after the 4-wide loads
+    for pos in (None, 0, 1, 5002, 10003, 10006):
+        for bad in (np.nan, np.inf, -np.inf):
+            x = base.copy()
+            if pos is not None:
+                x[pos] = bad
+            if dtype == "bfloat16":
+                host = IF.bf16(x.astype(np.float32))
+            elif dtype == "uint8":
+                host = (np.nan_to_num(x, nan=0, posinf=1, neginf=0) * 255).astype(np.uint8)
+            else:
+                host = x.astype(dtype)
+            d = IF.DeviceArray.from_host(host)
+            want = dtype == "uint8" or pos is None
+            assert _device_all_finite(d) == want, (dtype, pos, bad)
+            if pos is None:
+                break
+    # the default algorithm then takes FIR instead of FFT, as src/imfilter.jl:1200-1209
+    img = rng.random((64, 64)).astype(np.float32)
+    img[3, 4] = np.nan
+    k = IF.centered(rng.standard_normal((7, 7)))   # 49 taps, not separable: FFT when finite
+    assert IF.filter_algorithm(IF.DeviceArray.from_host(img), IF.factorkernel(k)) == IF.Algorithm.FIR()
+    img[3, 4] = 0.5
+    assert IF.filter_algorithm(IF.DeviceArray.from_host(img), IF.factorkernel(k)) == IF.Algorithm.FFT()

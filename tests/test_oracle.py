@@ -335,3 +335,58 @@ def test_minmax_generic_path_vs_scipy():
         mn, mx = O.minmax(a, [-h, -h], [w, w], ("fill", 0.5))
         np.testing.assert_array_equal(mn, ndi.minimum_filter(a, size=w, mode="constant", cval=0.5))
         np.testing.assert_array_equal(mx, ndi.maximum_filter(a, size=w, mode="constant", cval=0.5))
+
+
+# ---- FFT restatement -------------------------------------------------------------------
+@pytest.mark.parametrize("style", ["replicate", "circular", "symmetric", "reflect", ("fill", 0.3)])
+def test_fft_restatement_equals_direct_2d_jl(style):
+    # test/2d.jl:69-75, :99-101 and test/nd.jl:77-81: FFT ≈ FIR for every
+    # border, here to rtol 1e-12 over centred, offset and one-sided kernels
+    rng = np.random.default_rng(61)
+    img = rng.random((61, 47))
+    for k, first in ((rng.standard_normal((15, 15)), (-7, -7)), (rng.standard_normal((3, 5)), (5, 2)),
+                     (rng.standard_normal((31, 1)), (-15, 0)), (rng.standard_normal((2, 9)), (-3, 1))):
+        got = O.imfilter_fft(img, k, first, style)
+        want = O.imfilter_dense(img, k, list(first), style)
+        assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want)), (k.shape, first)
+    v = rng.random(100)
+    kk = np.ones(31) / 31
+    np.testing.assert_allclose(O.imfilter_fft(v, kk, (-15,), style), O.imfilter_dense(v, kk, [-15], style),
+                               rtol=1e-12, atol=1e-14)
+
+
+def test_fft_restatement_padded_size_is_nextprod():
+    # Pad(kernel, img, FFT()) pads to nextprod([2,3], n + K) (src/border.jl:240-248)
+    assert [O._nextprod23(n) for n in (4111, 4127, 9101, 27, 28)] == [4374, 4374, 9216, 27, 32]
+
+
+# ---- the -O3 -march=native OpenMP build the CPU baseline times ---------------------------
+def test_native_parallel_build_matches_portable_bit_for_bit():
+    import subprocess
+    repo = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run(["make", "-f", "oracle/Makefile", "native"], cwd=repo, capture_output=True, text=True)
+    if r.returncode != 0:
+        pytest.skip("no OpenMP toolchain: " + r.stderr[-200:])
+    rng = np.random.default_rng(62)
+    img = rng.random((40, 33, 6))
+    k = rng.standard_normal((5, 3, 2))
+    iir = [(0, [1.07757, -0.427266, 0.0650941], [1.07757, -0.427266, 0.0650941], 0.0809994, np.eye(3).ravel())]
+
+    def run():
+        return [O.imfilter_dense(img, k, [-2, -1, 0], "reflect"),
+                O.imfilter_factors(img, [(0, np.array([0.25, 0.5, 0.25]), -1), (2, np.array([1.0, -1.0]), 0)],
+                                   ("fill", 0.5)),
+                O.median(img, [-1, -2, 0], [3, 5, 1], "symmetric"),
+                O.minmax(img, [-1, -1, 0], [3, 4, 1], "circular")[0],
+                O.extrema_lemire(img, [3, 4, 1])[1],
+                O.imfilter_iir(img, iir, "replicate")]
+    base = run()
+    old = O._LIB
+    try:
+        assert O.use_library(O.NATIVE_PATH) >= 1
+        for n in (1, 3, 8):
+            O.set_threads(n)
+            for a, b in zip(base, run()):
+                np.testing.assert_array_equal(a, b)
+    finally:
+        O._LIB = old

@@ -69,6 +69,8 @@ def main():
                     help="single-thread CPU time of each other configuration's oracle sample")
     ap.add_argument("--no-extras", action="store_true", help="skip the other configurations (serving runs unless --serve-requests 0)")
     ap.add_argument("--serve-requests", type=int, default=20000)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the tile-shape autotuner (use the cached or default tile rules; profiling runs)")
     ap.add_argument("--profile-copy", action="store_true",
                     help="also run one plain copy of a known size (rocprof counter calibration)")
     args = ap.parse_args()
@@ -138,7 +140,7 @@ def main():
     serving = None
     del out
     tile_tune = None
-    if not args.no_extras or args.serve_requests > 0:
+    if (not args.no_extras or args.serve_requests > 0) and not args.no_autotune:
         # tile shapes of the fused separable kernel for this machine (the C1 extra
         # and the serving stream's separable requests use them)
         from imagefiltering import tiles

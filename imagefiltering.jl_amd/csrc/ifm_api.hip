@@ -334,6 +334,82 @@ int ifm_all_finite(ifm_ctx *ctx, const ifm_array *a, int *result) {
   return IFM_OK;
 }
 
+// Shapes of two arrays agree axis by axis and they share one dense layout.
+static int same_geometry(ifm_ctx *ctx, const ifm_array *a, const ifm_array *b, const Geo &g) {
+  if (a->ndim != b->ndim) return fail(ctx, IFM_E_DIM, "arrays differ in dimensions");
+  for (int d = 0; d < a->ndim; ++d)
+    if (a->size[d] != b->size[d]) return fail(ctx, IFM_E_DIM, "arrays differ in size");
+  if (!dense_under(*b, g)) return fail(ctx, IFM_E_UNSUPPORTED, "arrays must share one dense layout");
+  return IFM_OK;
+}
+
+int ifm_na_prepare(ifm_ctx *ctx, const ifm_array *in, ifm_array *tmp, ifm_array *valid, int *hasna) {
+  if (!ctx) return IFM_E_ARG;
+  int rc;
+  if ((rc = check_array(ctx, in, "in")) || (rc = check_array(ctx, tmp, "tmp")) || (rc = check_array(ctx, valid, "valid")))
+    return rc;
+  if (!hasna) return fail(ctx, IFM_E_ARG, "hasna is NULL");
+  if (!is_float_out(tmp->dtype) || !is_float_out(valid->dtype))
+    return fail(ctx, IFM_E_ARG, "tmp and valid must be floating point");
+  Geo g;
+  if (!layout_of(*in, g)) return fail(ctx, IFM_E_UNSUPPORTED, "in is not a dense array");
+  if ((rc = same_geometry(ctx, in, tmp, g)) || (rc = same_geometry(ctx, in, valid, g))) return rc;
+  View vi, vt, vv;
+  if ((rc = make_view(ctx, in, g, S_IN, true, vi))) return rc;
+  if ((rc = make_view(ctx, tmp, g, S_OUT, false, vt))) return rc;
+  if ((rc = make_view(ctx, valid, g, S_OUT2, false, vv))) return rc;
+  if (!ctx->finite_flag && hipMalloc(&ctx->finite_flag, sizeof(int)) != hipSuccess) {
+    ctx->finite_flag = nullptr;
+    return fail(ctx, IFM_E_NOMEM, "device allocation failed (flag)");
+  }
+  HIPCHK(hipMemsetAsync(ctx->finite_flag, 0, sizeof(int), ctx->stream));
+  HIPCHK(launch_na_prepare(vi.d, vt.d, vv.d, ctx->finite_flag, ctx->stream));
+  if ((rc = finish_view(ctx, vt)) || (rc = finish_view(ctx, vv))) return rc;
+  int h = 0;
+  HIPCHK(hipMemcpyAsync(&h, ctx->finite_flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  *hasna = h ? 1 : 0;
+  return IFM_OK;
+}
+
+int ifm_divide(ifm_ctx *ctx, ifm_array *num, const ifm_array *den) {
+  if (!ctx) return IFM_E_ARG;
+  int rc;
+  if ((rc = check_array(ctx, num, "num")) || (rc = check_array(ctx, den, "den"))) return rc;
+  if (!is_float_out(num->dtype)) return fail(ctx, IFM_E_ARG, "num must be floating point");
+  Geo g;
+  if (!layout_of(*num, g)) return fail(ctx, IFM_E_UNSUPPORTED, "num is not a dense array");
+  if ((rc = same_geometry(ctx, num, den, g))) return rc;
+  View vn, vd;
+  if ((rc = make_view(ctx, num, g, S_OUT, true, vn))) return rc;
+  if ((rc = make_view(ctx, den, g, S_IN, true, vd))) return rc;
+  HIPCHK(launch_divide(vn.d, vd.d, ctx->stream));
+  if ((rc = finish_view(ctx, vn))) return rc;
+  return sync_if_host(ctx, !num->device || !den->device);
+}
+
+int ifm_copy_into(ifm_ctx *ctx, const ifm_array *src, ifm_array *dst, const int64_t *dst_first) {
+  if (!ctx) return IFM_E_ARG;
+  int rc;
+  if ((rc = check_array(ctx, src, "src")) || (rc = check_array(ctx, dst, "dst"))) return rc;
+  if (src->ndim != dst->ndim) return fail(ctx, IFM_E_DIM, "src and dst differ in dimensions");
+  Geo g;
+  if (!layout_of(*dst, g)) return fail(ctx, IFM_E_UNSUPPORTED, "dst is not a dense array");
+  if (!dense_under(*src, g)) return fail(ctx, IFM_E_UNSUPPORTED, "src must share the layout of dst");
+  for (int d = 0; d < src->ndim; ++d) {
+    const int64_t f = dst_first ? dst_first[d] : 0;
+    if (f < 0 || f + src->size[d] > dst->size[d]) return fail(ctx, IFM_E_DIM, "block lies outside dst");
+  }
+  View vs, vd;
+  if ((rc = make_view(ctx, src, g, S_IN, true, vs))) return rc;
+  if ((rc = make_view(ctx, dst, g, S_OUT, true, vd))) return rc;
+  int64_t df[MAXD];
+  to_canon(g, dst_first, 0, df);
+  HIPCHK(launch_copy_into(vs.d, vd.d, df, ctx->stream));
+  if ((rc = finish_view(ctx, vd))) return rc;
+  return sync_if_host(ctx, !src->device || !dst->device);
+}
+
 int ifm_ctx_destroy(ifm_ctx *ctx) {
   if (!ctx) return IFM_OK;
   (void)hipStreamSynchronize(ctx->stream);

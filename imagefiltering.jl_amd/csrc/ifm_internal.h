@@ -329,6 +329,12 @@ hipError_t launch_fft_tile(const Dense4 &in, Dense4 &out, const int64_t *out_fir
                            const float2 *spec, const float2 *tw32, int *nanflag, int gen, int style, float fill,
                            hipStream_t st);
 
+// Elementwise passes (k_elem.hip): NaN mask and zero-filled copy (hasna set
+// when a NaN is met), num /= den, and a dense src copied into dst at dst_first.
+hipError_t launch_na_prepare(const Dense4 &in, Dense4 &tmp, Dense4 &valid, int *hasna, hipStream_t st);
+hipError_t launch_divide(Dense4 &num, const Dense4 &den, hipStream_t st);
+hipError_t launch_copy_into(const Dense4 &src, Dense4 &dst, const int64_t *dst_first, hipStream_t st);
+
 // Batched serving.  Each kernel takes an array of per-request descriptors in
 // device memory, the request of every tile (tile_req) and the first tile of
 // every request (tile_first); requests are 2-D images filtered over their

@@ -309,26 +309,17 @@ def _run(alg, out, img, kernel, border, out_first):
         _lib.check(L.ifm_imfilter_fir(ctx, ai, ao, of, fa, 0, b))
         return out
     kinds = {c.kind for c in comps}
+    if "iir" in kinds and kinds != {"iir"}:
+        return _run_mixed(out, img, comps, border, out_first)
     if isinstance(alg, IIR) or kinds == {"iir"}:
         if kinds != {"iir"}:
-            raise NotImplementedError("IIR algorithm with FIR factors (Algorithm.Mixed) is not implemented")
+            raise ArgumentError("Algorithm.IIR() needs IIR kernels only")
         if isinstance(border, Fill):
             # leftborder!/rightborder! convert the Fill value to the eltype of
             # the padded array, i.e. the output eltype (src/imfilter.jl:1027-1029)
             b.fill = convert_fill(border.value, eltype_of(out))
-        arr = (_lib.IfmIir * len(comps))()
-        for i, c in enumerate(comps):
-            k = c.iir
-            for j in range(3):
-                arr[i].a[j], arr[i].b[j] = float(k.a[j]), float(k.b[j])
-            arr[i].scale = k.scale
-            for j in range(9):
-                arr[i].M[j] = float(np.asarray(k.M).ravel()[j])
-            arr[i].axis = c.axis
-        _lib.check(L.ifm_imfilter_iir(ctx, ai, ao, arr, len(comps), b))
+        _lib.check(L.ifm_imfilter_iir(ctx, ai, ao, _iir_struct(comps), len(comps), b))
         return out
-    if "iir" in kinds:
-        raise NotImplementedError("mixed IIR/FIR cascades (Algorithm.Mixed) are not implemented")
     if isinstance(alg, FFT):
         k = kernelconv(comps, N)
         kk = _kernel_struct(k)
@@ -351,6 +342,134 @@ def _run(alg, out, img, kernel, border, out_first):
         return out
     k = kernelconv(comps, N)
     _lib.check(L.ifm_imfilter_dense(ctx, ai, ao, of, _kernel_struct(k), b, _lib.PREC_AUTO))
+    return out
+
+
+def _run_na(alg, out, img, kernel, border):
+    """imfilter(img, kernel, NA()): filter the image with its NaNs set to zero
+    and the mask of valid pixels, both with Fill(0), and divide
+    (imfilter_na_inseparable!, src/imfilter.jl:1109-1122).  Without NaNs and a
+    separable kernel the reference divides by per-axis filtered ones instead
+    (imfilter_na_separable!, :1124-1128); the filtered all-ones mask is the
+    product of those factors, so the result agrees up to rounding."""
+    if border.na is not None and border.na is not np.isnan:
+        raise NotImplementedError("NA with a predicate other than isnan")
+    import ctypes
+    T = eltype_of(out)
+    L = _lib.load()
+    dimg = img if isinstance(img, DeviceArray) else DeviceArray.from_host(img)
+    tmp = DeviceArray(img.shape, T, dimg.order)
+    valid = DeviceArray(img.shape, T, dimg.order)
+    hasna = ctypes.c_int(0)
+    _lib.check(L.ifm_na_prepare(_lib.ctx(), ifm_array(dimg), ifm_array(tmp), ifm_array(valid), ctypes.byref(hasna)))
+    _run(alg, out, tmp, kernel, Fill(0), None)
+    vp = DeviceArray(img.shape, T, dimg.order)
+    _run(alg, vp, valid, kernel, Fill(0), None)
+    _lib.check(L.ifm_divide(_lib.ctx(), ifm_array(out), ifm_array(vp)))
+    return out
+
+
+def _iir_struct(comps):
+    arr = (_lib.IfmIir * len(comps))()
+    for i, c in enumerate(comps):
+        k = c.iir
+        for j in range(3):
+            arr[i].a[j], arr[i].b[j] = float(k.a[j]), float(k.b[j])
+        arr[i].scale = k.scale
+        for j in range(9):
+            arr[i].M[j] = float(np.asarray(k.M).ravel()[j])
+        arr[i].axis = c.axis
+    return arr
+
+
+def _stage_axes(c, N):
+    """axes(kernel) of one cascade stage as (first, last) per dimension; an
+    IIR stage has axes 0:0 (src/kernelfactors.jl:508-509)."""
+    ax = [(0, 0)] * N
+    if c.kind == "fir":
+        ax[c.axis] = (c.first, c.first + len(c.taps) - 1)
+    elif c.kind == "dense":
+        ax = [(c.dense.first[d], c.dense.first[d] + c.dense.shape[d] - 1) for d in range(N)]
+    return ax
+
+
+def _run_mixed(out, img, comps, border, out_first):
+    """A cascade of IIR and FIR stages (Algorithm.Mixed), scheduled as the
+    reference's generic cascade does (src/imfilter.jl:372-457): the image is
+    padded once with the accumulated padding, doubled because IIR and FIR are
+    mixed (src/border.jl:614-619); each stage then filters the previous
+    stage's region shrunk by its own axes, an IIR stage over the whole region
+    it is given with its edge initialised from that region's ends (or the
+    Fill value), and the last stage is cut to the output axes.  Every stage
+    runs on the GPU; intermediates stay in HBM in the output eltype."""
+    N = len(img.shape)
+    T = eltype_of(out)
+    L = _lib.load()
+    ctx = _lib.ctx()
+    of = list(out_first) if out_first is not None else [0] * N
+    on = list(np.shape(out.data if isinstance(out, OffsetArray) else out))
+    inds = [(of[d], of[d] + on[d] - 1) for d in range(N)]
+    axes = [_stage_axes(c, N) for c in comps]
+    pf = [sum(a[d][0] for a in axes) for d in range(N)]
+    pl = [sum(a[d][1] for a in axes) for d in range(N)]
+    pf = [2 * f if f < 0 else f for f in pf]                    # doublepadding
+    pl = [2 * l if l > 0 else l for l in pl]
+    lo = [max(-f, 0) for f in pf]
+    hi = [max(l, 0) for l in pl]
+    dimg = img if isinstance(img, DeviceArray) else DeviceArray.from_host(img)
+    if isinstance(border, NoPad):
+        # imfilter!(r, out, A, kernel, NoPad(), inds): A is already padded and
+        # out_first is in A's own indices
+        A = dimg
+        cov = [(0, img.shape[d] - 1) for d in range(N)]
+        border = border.border if border.border is not None else Pad("replicate")
+    else:
+        if isinstance(border, Fill):
+            pb = Fill(border.value, tuple(lo), tuple(hi))
+        else:
+            pb = Pad(border.style if isinstance(border, Pad) else "replicate", tuple(lo), tuple(hi))
+        from .border import padarray
+        A = padarray(T, dimg, pb).data                         # covers [-lo, n + hi)
+        cov = [(-lo[d], img.shape[d] + hi[d] - 1) for d in range(N)]
+    region = [(inds[d][0] + pf[d], inds[d][1] + pl[d]) for d in range(N)]   # expand(inds, padding)
+    b_iir = _border_struct(border, T)
+    if isinstance(border, Fill):
+        b_iir.fill = convert_fill(border.value, T)
+    b_rep = _lib.IfmBorder()
+    b_rep.style = _lib.REPLICATE                               # no read leaves the buffer
+    nst = len(comps)
+    for s, c in enumerate(comps):
+        last = s == nst - 1
+        region = [(region[d][0] - axes[s][d][0], region[d][1] - axes[s][d][1]) for d in range(N)]   # shrink
+        if c.kind == "iir":
+            src = A
+            if region != cov:                                  # the stage filters its own region only
+                src = DeviceArray([r1 - r0 + 1 for r0, r1 in region], T, A.order)
+                fa0 = (_lib.IfmFactor * 1)()
+                _lib.check(L.ifm_imfilter_fir(ctx, ifm_array(A), ifm_array(src),
+                                              _lib.i64([region[d][0] - cov[d][0] for d in range(N)]), fa0, 0,
+                                              b_rep))
+            dst = DeviceArray(src.shape, T, A.order)
+            _lib.check(L.ifm_imfilter_iir(ctx, ifm_array(src), ifm_array(dst), _iir_struct([c]), 1, b_iir))
+            A, cov = dst, region
+            if last:
+                fa0 = (_lib.IfmFactor * 1)()
+                _lib.check(L.ifm_imfilter_fir(ctx, ifm_array(A), ifm_array(out),
+                                              _lib.i64([inds[d][0] - cov[d][0] for d in range(N)]), fa0, 0, b_rep))
+            continue
+        target = [(max(region[d][0], inds[d][0]), min(region[d][1], inds[d][1])) for d in range(N)] if last else region
+        dst = out if last else DeviceArray([r1 - r0 + 1 for r0, r1 in target], T, A.order)
+        ofs = _lib.i64([target[d][0] - cov[d][0] for d in range(N)])
+        if c.kind == "fir":
+            fa = (_lib.IfmFactor * 1)()
+            t = np.ascontiguousarray(c.taps, dtype=np.float64)
+            fa[0].axis, fa[0].len, fa[0].first = c.axis, len(t), c.first
+            fa[0].taps = t.ctypes.data_as(_lib.C.POINTER(_lib.C.c_double))
+            _lib.check(L.ifm_imfilter_fir(ctx, ifm_array(A), ifm_array(dst), ofs, fa, 1, b_rep))
+        else:
+            _lib.check(L.ifm_imfilter_dense(ctx, ifm_array(A), ifm_array(dst), ofs,
+                                            _kernel_struct(_comp_dense(c, N)), b_rep, _lib.PREC_AUTO))
+        A, cov = dst, target
     return out
 
 
@@ -402,10 +521,12 @@ def imfilter(*args):
         T = filter_type(S, kernel)
     if T.code is None or T in (UInt8, N0f8):
         raise NotImplementedError("integer output eltype %s: pass a floating-point T" % T)
+    img = _library_input(img)
     border = borderinstance(border)
-    if isinstance(border, NA):
-        raise NotImplementedError("NA() borders are not implemented")
     N = len(img.shape)
+    if isinstance(border, NA):
+        out = empty_like(img, T)
+        return _run_na(_choose(r, alg, img, kernel), out, img, kernel, border)
     if isinstance(border, Inner):
         lo, hi = _inner_padding(border, kernel, N)
         shape = tuple(n - l - h for n, l, h in zip(img.shape, lo, hi))
@@ -463,13 +584,16 @@ def imfilter_(*args):
     if isinstance(kernel, TriggsSdika):
         kernel = (kernel,)
     kernel = _processed(kernel)
-    img = img if isinstance(img, DeviceArray) else host_dense(img)
+    img = _library_input(img if isinstance(img, DeviceArray) else host_dense(img))
     N = len(img.shape)
     if isinstance(border, NoPad):
         return _nopad(out, img, kernel, rest[1] if len(rest) >= 2 else None, r)
     border = borderinstance(border)
     if isinstance(border, NA):
-        raise NotImplementedError("NA() borders are not implemented")
+        if tuple(np.shape(_outdata(out))) != tuple(img.shape):
+            raise DimensionMismatch("output axes %r disagree with the image axes %r" % (np.shape(out), img.shape))
+        _run_na(_choose(r, alg, img, kernel), _outdata(out), img, kernel, border)
+        return out
     if isinstance(border, Inner):
         lo, hi = _inner_padding(border, kernel, N)
         _run(_choose(r, alg, img, kernel), _outdata(out), img, kernel, Pad("replicate"), lo)
@@ -480,6 +604,17 @@ def imfilter_(*args):
         raise DimensionMismatch("output axes %r disagree with the image axes %r" % (np.shape(out), img.shape))
     _run(_choose(r, alg, img, kernel), _outdata(out), img, kernel, border, None)
     return out
+
+
+def _library_input(img):
+    """Integer images other than UInt8 (Int8 ... Int64, UInt16, UInt32, Bool)
+    enter the library as Float64, the type their products with floating-point
+    taps are computed in (an exact conversion below 2^53)."""
+    if isinstance(img, (DeviceArray, OffsetArray)):
+        return img
+    if eltype_of(img).code is None:
+        return np.asarray(img, dtype=np.float64)
+    return img
 
 
 def _outdata(out):
@@ -504,10 +639,17 @@ def _nopad(out, A, kernel, inds, r):
         if a_first[d] > inds[d].start + f or a_first[d] + np.shape(A_data)[d] - 1 < inds[d].stop - 1 + l:
             raise DimensionMismatch("requested indices and kernel indices do not agree with indices of padded input")
     full = all(inds[d].start == o_first[d] and len(inds[d]) == np.shape(o_data)[d] for d in range(N))
-    if not full:
-        raise NotImplementedError("NoPad with a sub-range of the output")
     of = [inds[d].start - a_first[d] for d in range(N)]
     alg = r.settings if r is not None else FIR()
-    _run(alg, o_data, host_dense(A_data) if not isinstance(A_data, DeviceArray) else A_data, kernel,
-         Pad("replicate"), of)
+    src = host_dense(A_data) if not isinstance(A_data, DeviceArray) else A_data
+    if full:
+        _run(alg, o_data, src, kernel, NoPad(Pad("replicate")), of)
+        return out
+    # only inds is written: filter into a block of that size, then copyto!
+    # it into out (the rest of out keeps its values)
+    T = eltype_of(o_data)
+    blk = empty_like(o_data, T, [len(r_) for r_ in inds])
+    _run(alg, blk, src, kernel, NoPad(Pad("replicate")), of)
+    _lib.check(_lib.load().ifm_copy_into(_lib.ctx(), ifm_array(blk), ifm_array(o_data),
+                                         _lib.i64([inds[d].start - o_first[d] for d in range(N)])))
     return out

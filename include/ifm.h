@@ -186,6 +186,20 @@ int ifm_imfilter_fft(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const in
 int ifm_imfilter_iir(ifm_ctx *ctx, const ifm_array *in, ifm_array *out,
                      const ifm_iir *factors, int nfactors, const ifm_border *border);
 
+/* ---- NA() borders and sub-ranges ------------------------------------------ */
+/* The elementwise steps of imfilter(img, kernel, NA()) as
+ * imfilter_na_inseparable! (src/imfilter.jl:1109-1122) takes them: tmp = img
+ * with NaN replaced by zero, valid = 0 where img is NaN and 1 elsewhere (both
+ * in their own floating-point eltype), *hasna = any(isnan, img).  The host
+ * then filters tmp and valid with Fill(0) and divides. */
+int ifm_na_prepare(ifm_ctx *ctx, const ifm_array *in, ifm_array *tmp, ifm_array *valid, int *hasna);
+/* num ./= den elementwise in FP64 (out[I] /= vp[I], src/imfilter.jl:1119-1121). */
+int ifm_divide(ifm_ctx *ctx, ifm_array *num, const ifm_array *den);
+/* copyto!(dst, R, src, R) for the block R = dst_first .. dst_first + size(src)
+ * - 1 of dst (src/imfilter.jl:372-375, :433-434): the NoPad sub-range form of
+ * imfilter! and the IIR stages of mixed cascades.  Eltypes may differ (convert). */
+int ifm_copy_into(ifm_ctx *ctx, const ifm_array *src, ifm_array *dst, const int64_t *dst_first);
+
 /* ---- mapwindow ------------------------------------------------------------ */
 /* Replaces mapwindow(median!, img, window; border) -> median_fast!
  * (src/mapwindow.jl:123-131, :270-333): the ((L+1)÷2)-th smallest of the

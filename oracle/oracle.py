@@ -258,3 +258,62 @@ def imfilter_fft(img, k, kfirst, border):
     R = np.fft.irfftn(np.fft.rfftn(pad, axes=ax) * np.conj(np.fft.rfftn(kp, axes=ax)), s=N, axes=ax)
     sl = tuple(slice(lo[d], lo[d] + a.shape[d]) for d in range(a.ndim))
     return np.asfortranarray(R[sl])
+
+
+def imfilter_na(img, stage_fn):
+    """NA() border restated from imfilter_na_inseparable! (src/imfilter.jl:
+    1109-1122): stage_fn(x, border) filters x with the kernel under a border;
+    the image with NaN set to zero and the mask of valid pixels are filtered
+    with Fill(0) and divided."""
+    a = np.asarray(img, dtype=np.float64)
+    nan = np.isnan(a)
+    num = stage_fn(np.where(nan, 0.0, a), ("fill", 0.0))
+    den = stage_fn((~nan).astype(np.float64), ("fill", 0.0))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return num / den
+
+
+def imfilter_mixed(img, stages, border):
+    """A cascade of FIR and IIR stages as the reference's generic cascade runs
+    it (src/imfilter.jl:372-457): pad once with the accumulated padding,
+    doubled when IIR and FIR stages are mixed (src/border.jl:614-619); each
+    stage filters the previous region shrunk by its own axes (IIR: 0:0), an
+    IIR stage over its whole region with its edge initialised from that
+    region's ends (or the Fill value); the last stage is cut to the image.
+    stages: ("fir", axis, taps, first) | ("iir", axis, a, b, scale, M)."""
+    a = np.asarray(img, dtype=np.float64)
+    N = a.ndim
+    ax = []
+    for st in stages:
+        t = [(0, 0)] * N
+        if st[0] == "fir":
+            t[st[1]] = (st[3], st[3] + len(st[2]) - 1)
+        ax.append(t)
+    pf = [sum(t[d][0] for t in ax) for d in range(N)]
+    pl = [sum(t[d][1] for t in ax) for d in range(N)]
+    if any(s[0] == "iir" for s in stages) and any(s[0] == "fir" for s in stages):
+        pf = [2 * f if f < 0 else f for f in pf]
+        pl = [2 * l if l > 0 else l for l in pl]
+    lo = [max(-f, 0) for f in pf]
+    hi = [max(l, 0) for l in pl]
+    A = padarray(a, border, lo, hi)
+    cov = [(-lo[d], a.shape[d] + hi[d] - 1) for d in range(N)]
+    inds = [(0, a.shape[d] - 1) for d in range(N)]
+    region = [(inds[d][0] + pf[d], inds[d][1] + pl[d]) for d in range(N)]
+    for s, st in enumerate(stages):
+        last = s == len(stages) - 1
+        region = [(region[d][0] - ax[s][d][0], region[d][1] - ax[s][d][1]) for d in range(N)]
+        if st[0] == "iir":
+            sl = tuple(slice(region[d][0] - cov[d][0], region[d][1] - cov[d][0] + 1) for d in range(N))
+            A = imfilter_iir(np.asfortranarray(A[sl]), [st[1:]], border if not isinstance(border, str) else "replicate")
+            cov = region
+            continue
+        target = [(max(region[d][0], inds[d][0]), min(region[d][1], inds[d][1])) for d in range(N)] if last else region
+        A = imfilter_factors(A, [(st[1], np.asarray(st[2], dtype=np.float64), st[3])], "replicate",
+                             [target[d][0] - cov[d][0] for d in range(N)],
+                             [target[d][1] - target[d][0] + 1 for d in range(N)])
+        cov = target
+    if stages and stages[-1][0] == "iir":
+        sl = tuple(slice(inds[d][0] - cov[d][0], inds[d][1] - cov[d][0] + 1) for d in range(N))
+        A = A[sl]
+    return np.asfortranarray(A)

@@ -390,3 +390,66 @@ def test_native_parallel_build_matches_portable_bit_for_bit():
                 np.testing.assert_array_equal(a, b)
     finally:
         O._LIB = old
+
+
+# ---- NA() and mixed IIR / FIR cascades -------------------------------------------------
+def _factor_fn(taps, first, axis=0):
+    return lambda x, b: O.imfilter_factors(x, [(axis, np.asarray(taps, dtype=np.float64), first)], b)
+
+
+def test_na_reference_vectors_border_jl():
+    # test/border.jl:272-284
+    got = O.imfilter_na(np.arange(1, 11.0), _factor_fn([1 / 3] * 3, -1))
+    np.testing.assert_allclose(got, [1.5] + list(range(2, 10)) + [9.5], rtol=1e-14)
+    x = np.array([1, np.nan, np.inf, 0, -np.inf, np.inf, np.nan, np.nan, 1])
+    got = O.imfilter_na(x, _factor_fn([1.0, 1.0], 0))
+    want = np.array([1, np.inf, np.inf, -np.inf, np.nan, np.inf, np.nan, 1, 1])
+    np.testing.assert_array_equal(np.isnan(got), np.isnan(want))
+    np.testing.assert_array_equal(got[~np.isnan(want)], want[~np.isnan(want)])
+    np.testing.assert_array_equal(O.imfilter_na(np.arange(1, 6.0), _factor_fn([1.0], 0)), np.arange(1, 6.0))
+
+
+def test_na_target_2d_jl():
+    # test/2d.jl:174-181: impulse at (1,2) with a kernel on axes (-1:0, 1:2)
+    img = np.zeros((5, 7))
+    img[0, 1] = 1
+    k = np.array([[0.1, 0.2], [0.4, 0.5]])
+    got = O.imfilter_na(img, lambda x, b: O.imfilter_dense(x, k, [-1, 1], b))
+    want = np.zeros((5, 7))
+    want[0, 0] = k[1, 0] / (k[1, 0] + k[1, 1])
+    want[1, 0] = k[0, 0] / k.sum()
+    want[:, 6] = np.nan
+    np.testing.assert_array_equal(np.isnan(got), np.isnan(want))
+    np.testing.assert_allclose(got[:, :6], want[:, :6], atol=1e-15)
+
+
+def _iir_stage(axis, sigma):
+    import imagefiltering as IF
+    k = IF.KernelFactors.IIRGaussian(sigma)
+    return ("iir", axis, k.a, k.b, k.scale, k.M)
+
+
+@pytest.mark.parametrize("border", ["replicate", ("fill", 0.0)])
+def test_mixed_cascade_commutes_triggs_jl(border):
+    # test/triggs.jl:34-41: (IIRGaussian(2), box 3) ≈ (box 3, IIRGaussian(2)) on 1:8
+    img = np.arange(1, 9.0)
+    box = ("fir", 0, [1 / 3] * 3, -1)
+    a = O.imfilter_mixed(img, [_iir_stage(0, 2.0), box], border)
+    b = O.imfilter_mixed(img, [box, _iir_stage(0, 2.0)], border)
+    np.testing.assert_allclose(a, b, rtol=1e-8)
+    # and it is not the unpadded composition: the IIR ran over the doubled padding
+    assert a.shape == img.shape
+
+
+def test_mixed_cascade_replicate_equals_long_constant_extension():
+    # with "replicate" the padded cascade equals the same stages on a long
+    # edge-extended signal, cut back to the image (the IIR's init is exact for
+    # a constant continuation)
+    rng = np.random.default_rng(63)
+    img = rng.random(40)
+    stages = [_iir_stage(0, 1.5), ("fir", 0, [0.25, 0.5, 0.25], -1), ("fir", 0, [1.0, -1.0], 0)]
+    got = O.imfilter_mixed(img, stages, "replicate")
+    ext = np.concatenate([np.full(400, img[0]), img, np.full(400, img[-1])])
+    x = O.imfilter_iir(ext, [stages[0][1:]], "replicate")
+    x = O.imfilter_factors(x, [(0, np.array(stages[1][2]), -1), (0, np.array(stages[2][2]), 0)], "replicate")
+    np.testing.assert_allclose(got, x[400:440], rtol=1e-9, atol=1e-12)

@@ -410,6 +410,33 @@ int ifm_copy_into(ifm_ctx *ctx, const ifm_array *src, ifm_array *dst, const int6
   return sync_if_host(ctx, !src->device || !dst->device);
 }
 
+int ifm_local_extrema(ifm_ctx *ctx, const ifm_array *in, ifm_array *flags, const int64_t *window,
+                      const int32_t *edges, int sense) {
+  if (!ctx) return IFM_E_ARG;
+  int rc;
+  if ((rc = check_array(ctx, in, "in")) || (rc = check_array(ctx, flags, "flags"))) return rc;
+  if (flags->dtype != IFM_U8) return fail(ctx, IFM_E_ARG, "flags must be UInt8");
+  if (sense != 1 && sense != -1) return fail(ctx, IFM_E_ARG, "sense must be +1 or -1");
+  Geo g;
+  if (!layout_of(*in, g)) return fail(ctx, IFM_E_UNSUPPORTED, "in is not a dense array");
+  if ((rc = same_geometry(ctx, in, flags, g))) return rc;
+  int half[MAXD] = {0, 0, 0, 0}, edg[MAXD] = {1, 1, 1, 1};
+  for (int m = 0; m < MAXD; ++m) {
+    const int d = g.perm[m];
+    if (d < 0 || d >= in->ndim) continue;
+    const int64_t w = window ? window[d] : 3;
+    if (w < 1 || w % 2 != 1 || w > 63) return fail(ctx, IFM_E_ARG, "window entries must be odd and at most 63");
+    half[m] = (int)(w >> 1);
+    edg[m] = edges ? (edges[d] != 0) : 1;
+  }
+  View vi, vf;
+  if ((rc = make_view(ctx, in, g, S_IN, true, vi))) return rc;
+  if ((rc = make_view(ctx, flags, g, S_OUT, false, vf))) return rc;
+  HIPCHK(launch_local_extrema(vi.d, (uint8_t *)vf.d.data, half, edg, sense, ctx->stream));
+  if ((rc = finish_view(ctx, vf))) return rc;
+  return sync_if_host(ctx, !in->device || !flags->device);
+}
+
 int ifm_ctx_destroy(ifm_ctx *ctx) {
   if (!ctx) return IFM_OK;
   (void)hipStreamSynchronize(ctx->stream);

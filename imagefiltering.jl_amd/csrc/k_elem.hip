@@ -126,3 +126,70 @@ hipError_t launch_copy_into(const Dense4 &src, Dense4 &dst, const int64_t *dst_f
 }
 
 }  // namespace ifm
+
+namespace ifm {
+
+// findlocalmaxima / findlocalminima (src/extrema.jl:131-163): flag[i] = 1 when
+// i lies in the clipped range (edges[d] == 0 drops the first and last index
+// of axis d) and img[i] beats every in-bounds neighbour of its window (>
+// for maxima, < for minima, compared as the element type's values).
+struct LocalExtArgs {
+  const void *in;
+  uint8_t *flag;
+  int64_t n[MAXD];
+  int half[MAXD];
+  int edges[MAXD];
+  int dt;
+  int sense;  // +1 maxima, -1 minima
+  int64_t count;
+};
+
+__global__ void __launch_bounds__(256) k_local_extrema(LocalExtArgs a) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < a.count;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int64_t c[MAXD], r = i;
+    bool ok = true;
+#pragma unroll
+    for (int d = 0; d < MAXD; ++d) {
+      c[d] = r % a.n[d];
+      r /= a.n[d];
+      if (!a.edges[d] && (c[d] == 0 || c[d] == a.n[d] - 1)) ok = false;
+    }
+    if (ok) {
+      const double v = load_d(a.in, i, a.dt);
+      for (int j3 = -a.half[3]; j3 <= a.half[3] && ok; ++j3)
+        for (int j2 = -a.half[2]; j2 <= a.half[2] && ok; ++j2)
+          for (int j1 = -a.half[1]; j1 <= a.half[1] && ok; ++j1)
+            for (int j0 = -a.half[0]; j0 <= a.half[0] && ok; ++j0) {
+              if ((j0 | j1 | j2 | j3) == 0) continue;
+              const int64_t q0 = c[0] + j0, q1 = c[1] + j1, q2 = c[2] + j2, q3 = c[3] + j3;
+              if (q0 < 0 || q0 >= a.n[0] || q1 < 0 || q1 >= a.n[1] || q2 < 0 || q2 >= a.n[2] || q3 < 0 ||
+                  q3 >= a.n[3])
+                continue;
+              const double w = load_d(a.in, q0 + a.n[0] * (q1 + a.n[1] * (q2 + a.n[2] * q3)), a.dt);
+              ok = a.sense > 0 ? (v > w) : (v < w);
+            }
+    }
+    a.flag[i] = ok ? 1 : 0;
+  }
+}
+
+hipError_t launch_local_extrema(const Dense4 &in, uint8_t *flag, const int *half, const int *edges, int sense,
+                                hipStream_t st) {
+  LocalExtArgs a;
+  a.in = in.data;
+  a.flag = flag;
+  for (int d = 0; d < MAXD; ++d) {
+    a.n[d] = in.n[d];
+    a.half[d] = half[d];
+    a.edges[d] = edges[d];
+  }
+  a.dt = in.dtype;
+  a.sense = sense;
+  a.count = in.count();
+  if (a.count == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_local_extrema, dim3(elem_blocks(a.count)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+}  // namespace ifm

@@ -22,9 +22,11 @@ from .imfilter import (CPU1, FFT, FIR, IIR, AbstractResource, Algorithm, CPUThre
 from .kernelfactors import kernelfactors
 from .mapwindow import extrema, mapwindow, mapwindow_, maximum, median, median_, minimum, resolve_window
 from .offsetarrays import OffsetArray, centered
+from .specialty import BlobLoG, blob_LoG, findlocalmaxima, findlocalminima, imgradients, multiLoG
 
 __all__ = [
     "Kernel", "KernelFactors", "Pad", "Fill", "Inner", "NA", "NoPad", "BorderArray", "Algorithm",
     "imfilter", "imfilter_", "mapwindow", "mapwindow_", "padarray", "centered", "kernelfactors",
     "OffsetArray", "DeviceArray", "median", "median_", "minimum", "maximum", "extrema",
+    "imgradients", "findlocalmaxima", "findlocalminima", "blob_LoG", "multiLoG", "BlobLoG",
 ]

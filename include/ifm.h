@@ -200,6 +200,15 @@ int ifm_divide(ifm_ctx *ctx, ifm_array *num, const ifm_array *den);
  * imfilter! and the IIR stages of mixed cascades.  Eltypes may differ (convert). */
 int ifm_copy_into(ifm_ctx *ctx, const ifm_array *src, ifm_array *dst, const int64_t *dst_first);
 
+/* findlocalmaxima (sense = +1) / findlocalminima (sense = -1)
+ * (src/extrema.jl:122-163): flags[i] = 1 where img[i] is strictly greater
+ * (smaller) than every in-bounds element of the odd window around it
+ * (window[d] <= 63; NULL: 3 per axis), excluding the first and last index of
+ * axis d when edges[d] == 0 (NULL: all edges included).  The caller turns the
+ * flags into the CartesianIndex list (the blob_LoG peaks). */
+int ifm_local_extrema(ifm_ctx *ctx, const ifm_array *in, ifm_array *flags, const int64_t *window,
+                      const int32_t *edges, int sense);
+
 /* ---- mapwindow ------------------------------------------------------------ */
 /* Replaces mapwindow(median!, img, window; border) -> median_fast!
  * (src/mapwindow.jl:123-131, :270-333): the ((L+1)÷2)-th smallest of the

@@ -410,6 +410,37 @@ int ifm_copy_into(ifm_ctx *ctx, const ifm_array *src, ifm_array *dst, const int6
   return sync_if_host(ctx, !src->device || !dst->device);
 }
 
+int ifm_convert_int(ifm_ctx *ctx, const ifm_array *src, void *dst, int dst_device, int bits, int is_signed,
+                    double scale, int *inexact) {
+  if (!ctx) return IFM_E_ARG;
+  int rc;
+  if ((rc = check_array(ctx, src, "src"))) return rc;
+  if (!dst || !inexact) return fail(ctx, IFM_E_ARG, "dst and inexact must not be NULL");
+  if (bits != 8 && bits != 16 && bits != 32 && bits != 64) return fail(ctx, IFM_E_ARG, "bits must be 8, 16, 32 or 64");
+  Geo g;
+  if (!layout_of(*src, g)) return fail(ctx, IFM_E_UNSUPPORTED, "src is not a dense array");
+  View vs;
+  if ((rc = make_view(ctx, src, g, S_IN, true, vs))) return rc;
+  const size_t bytes = (size_t)vs.d.count() * (size_t)(bits / 8);
+  void *d = dst;
+  if (!dst_device) {
+    d = slot_buffer(ctx, S_OUT, bytes);
+    if (!d) return fail(ctx, IFM_E_NOMEM, "device allocation failed (staging)");
+  }
+  if (!ctx->finite_flag && hipMalloc(&ctx->finite_flag, sizeof(int)) != hipSuccess) {
+    ctx->finite_flag = nullptr;
+    return fail(ctx, IFM_E_NOMEM, "device allocation failed (flag)");
+  }
+  HIPCHK(hipMemsetAsync(ctx->finite_flag, 0, sizeof(int), ctx->stream));
+  HIPCHK(launch_to_int(vs.d, d, bits, is_signed, scale, ctx->finite_flag, ctx->stream));
+  if (!dst_device && bytes) HIPCHK(hipMemcpyAsync(dst, d, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  int h = 0;
+  HIPCHK(hipMemcpyAsync(&h, ctx->finite_flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  *inexact = h ? 1 : 0;
+  return IFM_OK;
+}
+
 int ifm_local_extrema(ifm_ctx *ctx, const ifm_array *in, ifm_array *flags, const int64_t *window,
                       const int32_t *edges, int sense) {
   if (!ctx) return IFM_E_ARG;

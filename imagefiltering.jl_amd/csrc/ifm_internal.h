@@ -334,6 +334,10 @@ hipError_t launch_fft_tile(const Dense4 &in, Dense4 &out, const int64_t *out_fir
 hipError_t launch_na_prepare(const Dense4 &in, Dense4 &tmp, Dense4 &valid, int *hasna, hipStream_t st);
 hipError_t launch_divide(Dense4 &num, const Dense4 &den, hipStream_t st);
 hipError_t launch_copy_into(const Dense4 &src, Dense4 &dst, const int64_t *dst_first, hipStream_t st);
+// Floating-point values into an integer eltype (bits 8/16/32/64), *inexact set
+// for a non-integer or out-of-range value.
+hipError_t launch_to_int(const Dense4 &src, void *dst, int bits, int is_signed, double scale, int *inexact,
+                         hipStream_t st);
 // Local extrema flags (one byte per element, dense in canonical order).
 hipError_t launch_local_extrema(const Dense4 &in, uint8_t *flag, const int *half, const int *edges, int sense,
                                 hipStream_t st);

@@ -193,3 +193,69 @@ hipError_t launch_local_extrema(const Dense4 &in, uint8_t *flag, const int *half
 }
 
 }  // namespace ifm
+
+namespace ifm {
+
+// convert(T, x) of floating-point results into an integer eltype T of `bits`
+// width (signed or not), scaled by `scale` first (255 for N0f8 codes): a
+// value that is not an integer or lies outside T's range sets *inexact, as
+// Julia's convert throws InexactError (src/imfilter.jl:233-254).
+struct ToIntArgs {
+  const void *src;
+  void *dst;
+  int dt_src;
+  int bits, is_signed;
+  double scale;
+  int64_t count;
+  int *inexact;
+};
+
+__global__ void __launch_bounds__(256) k_to_int(ToIntArgs a) {
+  bool bad = false;
+  double lo, hi;
+  if (a.is_signed) {
+    hi = ldexp(1.0, a.bits - 1) - 1.0;
+    lo = -ldexp(1.0, a.bits - 1);
+  } else {
+    hi = ldexp(1.0, a.bits) - 1.0;
+    lo = 0.0;
+  }
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < a.count;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    // integers must be exact; a fixed-point target (scale != 1, N0f8) rounds
+    // to the nearest code as convert(N0f8, x) does and checks the range only
+    const double s = load_d(a.src, i, a.dt_src) * a.scale;
+    const double v = a.scale != 1.0 ? rint(s) : s;
+    const bool ok = v == rint(v) && v >= lo && v <= hi;  // NaN fails every comparison
+    bad |= !ok;
+    const double c = ok ? v : 0.0;
+    switch (a.bits) {
+    case 8: ((uint8_t *)a.dst)[i] = (uint8_t)(int)(int64_t)c; break;
+    case 16: ((uint16_t *)a.dst)[i] = (uint16_t)(int)(int64_t)c; break;
+    case 32: ((uint32_t *)a.dst)[i] = (uint32_t)(int64_t)c; break;
+    default:
+      if (a.is_signed) ((int64_t *)a.dst)[i] = (int64_t)c;
+      else ((uint64_t *)a.dst)[i] = (uint64_t)c;
+      break;
+    }
+  }
+  if (bad) atomicOr(a.inexact, 1);
+}
+
+hipError_t launch_to_int(const Dense4 &src, void *dst, int bits, int is_signed, double scale, int *inexact,
+                         hipStream_t st) {
+  ToIntArgs a;
+  a.src = src.data;
+  a.dst = dst;
+  a.dt_src = src.dtype;
+  a.bits = bits;
+  a.is_signed = is_signed;
+  a.scale = scale;
+  a.count = src.count();
+  a.inexact = inexact;
+  if (a.count == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_to_int, dim3(elem_blocks(a.count)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+}  // namespace ifm

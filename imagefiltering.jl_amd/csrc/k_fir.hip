@@ -67,8 +67,8 @@ __global__ void __launch_bounds__(256) k_pad(PadArgs a) {
 static uint64_t fill_bits_for(int dt, double v) {
   uint64_t b = 0;
   switch (dt) {
-  case IFM_U8:
-  case IFM_N0F8: b = (uint64_t)(uint8_t)(int)v; break;  // caller passes the code
+  case IFM_U8: b = (uint64_t)(uint8_t)(int)v; break;
+  case IFM_N0F8: b = (uint64_t)(uint8_t)(int)llrint(v * 255.0); break;  // the value code/255
   case IFM_F32: { float f = (float)v; uint32_t u; memcpy(&u, &f, 4); b = u; break; }
   case IFM_F64: memcpy(&b, &v, 8); break;
   case IFM_BF16: { float f = (float)v; uint32_t u; memcpy(&u, &f, 4); b = (u + 0x7fffu + ((u >> 16) & 1u)) >> 16; break; }

@@ -253,7 +253,8 @@ __device__ __forceinline__ KT window_key(const MedianAnyArgs &a, const int *q, c
       // fill already converted to the eltype; reuse the key function on a local
       float ff = (float)a.fill;
       uint32_t kk;
-      if (a.dt == IFM_U8 || a.dt == IFM_N0F8) kk = (uint32_t)(int)a.fill;
+      if (a.dt == IFM_U8) kk = (uint32_t)(int)a.fill;
+      else if (a.dt == IFM_N0F8) kk = (uint32_t)(int)rint(a.fill * 255.0);  // the value code/255
       else if (a.dt == IFM_F32) kk = key32(&ff, 0, IFM_F32);
       else if (a.dt == IFM_BF16) { uint16_t b = f32_to_bf16(ff); kk = key32(&b, 0, IFM_BF16); }
       else { __half h = __float2half_rn(ff); kk = key32(&h, 0, IFM_F16); }
@@ -305,8 +306,8 @@ __global__ void __launch_bounds__(256) k_median_any(MedianAnyArgs a) {
 static uint32_t fill_key32(int dt, double fill) {
   // host twin of key32 for the fill value
   switch (dt) {
-  case IFM_U8:
-  case IFM_N0F8: return (uint32_t)(int)fill;
+  case IFM_U8: return (uint32_t)(int)fill;
+  case IFM_N0F8: return (uint32_t)(int)llrint(fill * 255.0);  // the value code/255
   case IFM_F32: {
     float f = (float)fill;
     uint32_t u;

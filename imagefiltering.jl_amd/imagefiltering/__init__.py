@@ -17,8 +17,9 @@ from .device import DeviceArray
 from .eltypes import (BF16Array, BFloat16, Float16, Float32, Float64, N0f8, N0f8Array, UInt8, bf16,
                       bf16_to_float32, n0f8)
 from .errors import ArgumentError, DimensionMismatch, InexactError, MethodError
-from .imfilter import (CPU1, FFT, FIR, IIR, AbstractResource, Algorithm, CPUThreads, FIRTiled, HIPDevice, Mixed,
-                       factorkernel, filter_algorithm, filter_type, imfilter, imfilter_, kernelconv)
+from .imfilter import (CPU1, FFT, FIR, IIR, AbstractResource, Algorithm, ColorImage, CPUThreads, FIRTiled, HIPDevice,
+                       Mixed, channelview, colorview, factorkernel, filter_algorithm, filter_type, imfilter, imfilter_,
+                       kernelconv)
 from .kernelfactors import kernelfactors
 from .mapwindow import extrema, mapwindow, mapwindow_, maximum, median, median_, minimum, resolve_window
 from .offsetarrays import OffsetArray, centered
@@ -29,4 +30,5 @@ __all__ = [
     "imfilter", "imfilter_", "mapwindow", "mapwindow_", "padarray", "centered", "kernelfactors",
     "OffsetArray", "DeviceArray", "median", "median_", "minimum", "maximum", "extrema",
     "imgradients", "findlocalmaxima", "findlocalminima", "blob_LoG", "multiLoG", "BlobLoG",
+    "ColorImage", "colorview", "channelview",
 ]

@@ -78,7 +78,7 @@ MAX_TILE_RULES = 256
 EXPORTS = [
     "ifm_ctx_create", "ifm_ctx_destroy", "ifm_ctx_set_stream", "ifm_ctx_synchronize", "ifm_last_error",
     "ifm_version", "ifm_padarray", "ifm_imfilter_fir", "ifm_imfilter_fir_noalloc", "ifm_imfilter_dense", "ifm_all_finite", "ifm_na_prepare", "ifm_divide",
-    "ifm_copy_into", "ifm_local_extrema", "ifm_imfilter_fft",
+    "ifm_copy_into", "ifm_local_extrema", "ifm_convert_int", "ifm_imfilter_fft",
     "ifm_imfilter_iir", "ifm_mapwindow_median", "ifm_mapwindow_minmax", "ifm_extrema_filter",
     "ifm_batch_run", "ifm_serve", "ifm_fill_uniform",
     "ifm_device_alloc", "ifm_device_free", "ifm_memcpy_h2d", "ifm_memcpy_d2h", "ifm_memset_d",
@@ -119,6 +119,7 @@ def load(path=LIBPATH):
     L.ifm_na_prepare.argtypes = [vp, P(IfmArray), P(IfmArray), P(IfmArray), P(C.c_int)]
     L.ifm_divide.argtypes = [vp, P(IfmArray), P(IfmArray)]
     L.ifm_copy_into.argtypes = [vp, P(IfmArray), P(IfmArray), P(C.c_int64)]
+    L.ifm_convert_int.argtypes = [vp, P(IfmArray), vp, C.c_int, C.c_int, C.c_int, C.c_double, P(C.c_int)]
     L.ifm_local_extrema.argtypes = [vp, P(IfmArray), P(IfmArray), P(C.c_int64), P(C.c_int32), C.c_int]
     L.ifm_imfilter_iir.argtypes = [vp, P(IfmArray), P(IfmArray), P(IfmIir), C.c_int, P(IfmBorder)]
     L.ifm_mapwindow_median.argtypes = [vp, P(IfmArray), P(IfmArray), P(C.c_int64), P(C.c_int64), P(C.c_int64),

@@ -313,7 +313,7 @@ def convert_fill(value, et):
         if et is N0f8:
             if not 0.0 <= v <= 1.0:
                 raise ValueError
-            return float(round(v * 255))     # the code
+            return float(round(v * 255)) / 255.0   # the value code/255 (the library recovers the code)
         if et is BFloat16:
             from .eltypes import bf16, bf16_to_float32
             return float(bf16_to_float32(bf16(np.float32(v)))[()])
@@ -379,8 +379,12 @@ class BorderArray:
         if not isinstance(border, Inner):
             _padding_of(np.ndim(inner) if not isinstance(inner, DeviceArray) else inner.ndim, border)
         self.inner = inner
+        self._padding = border
         if isinstance(border, Fill):
-            border = Fill(convert_fill(border.value, eltype_of(inner)), border.lo, border.hi)
+            et = eltype_of(inner)
+            v = convert_fill(border.value, et)
+            # indexing returns stored elements: the code for N0f8
+            border = Fill(int(round(v * 255)) if et is N0f8 else v, border.lo, border.hi)
         self.border = border
 
     def axes(self):
@@ -421,4 +425,4 @@ class BorderArray:
     def copy(self):
         if isinstance(self.border, Inner):
             return np.array(self.inner, copy=True)
-        return padarray(self.inner, self.border)
+        return padarray(self.inner, self._padding)

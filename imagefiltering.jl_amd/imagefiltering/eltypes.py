@@ -27,13 +27,52 @@ Float16 = Eltype("Float16", _lib.F16, np.float16, "float")
 BFloat16 = Eltype("BFloat16", _lib.BF16, np.uint16, "float")
 Float32 = Eltype("Float32", _lib.F32, np.float32, "float")
 Float64 = Eltype("Float64", _lib.F64, np.float64, "float")
-Int64 = Eltype("Int64", None, np.int64, "int")   # kernels / promotion only
+# Integer eltypes other than UInt8 are not library storage types: images of
+# them enter the filters as Float64, and integer results leave through
+# ifm_convert_int (an exact conversion, or InexactError).
+Int8 = Eltype("Int8", None, np.int8, "int")
+Int16 = Eltype("Int16", None, np.int16, "int")
+Int32 = Eltype("Int32", None, np.int32, "int")
+Int64 = Eltype("Int64", None, np.int64, "int")
+UInt16 = Eltype("UInt16", None, np.uint16, "int")
+UInt32 = Eltype("UInt32", None, np.uint32, "int")
+UInt64 = Eltype("UInt64", None, np.uint64, "int")
 Bool = Eltype("Bool", None, np.bool_, "int")
 
 _BY_DTYPE = {np.dtype(np.uint8): UInt8, np.dtype(np.float16): Float16, np.dtype(np.float32): Float32,
-             np.dtype(np.float64): Float64, np.dtype(np.int64): Int64, np.dtype(np.int32): Int64,
-             np.dtype(np.int16): Int64, np.dtype(np.int8): Int64, np.dtype(np.uint16): Int64,
-             np.dtype(np.uint32): Int64, np.dtype(np.bool_): Bool}
+             np.dtype(np.float64): Float64, np.dtype(np.int64): Int64, np.dtype(np.int32): Int32,
+             np.dtype(np.int16): Int16, np.dtype(np.int8): Int8, np.dtype(np.uint16): UInt16,
+             np.dtype(np.uint32): UInt32, np.dtype(np.uint64): UInt64, np.dtype(np.bool_): Bool}
+
+# (bits, signed) of the integer eltypes
+INT_INFO = {"Bool": (1, False), "Int8": (8, True), "Int16": (16, True), "Int32": (32, True), "Int64": (64, True),
+            "UInt8": (8, False), "UInt16": (16, False), "UInt32": (32, False), "UInt64": (64, False)}
+_INT_BY = {v: k for k, v in INT_INFO.items() if k != "Bool"}
+
+
+def _int_eltype(bits, signed):
+    return globals()[_INT_BY[(bits, signed)]]
+
+
+def promote_int(a, b):
+    """promote_type for two integer eltypes: Bool yields to the other; of a
+    signed and an unsigned type the larger wins, the unsigned one on a tie."""
+    if a is b:
+        return a
+    if a is Bool:
+        return b
+    if b is Bool:
+        return a
+    (ba, sa), (bb, sb) = INT_INFO[a.name], INT_INFO[b.name]
+    if sa == sb:
+        return a if ba >= bb else b
+    s, u = (a, b) if sa else (b, a)
+    return s if INT_INFO[s.name][0] > INT_INFO[u.name][0] else u
+
+
+def signed_type(S):
+    """signed_type for the Laplacian's output (src/imfilter.jl:1143-1146)."""
+    return {"UInt8": Int16, "UInt16": Int32, "UInt32": Int64}.get(S.name, Int64)
 
 
 class N0f8Array(np.ndarray):
@@ -125,7 +164,9 @@ def mul_type(S, T):
         return S
     if S is N0f8:
         return Float32
-    return Int64
+    # zero(S) * zero(T) + zero(S) * zero(T): Bool + Bool is Int
+    R = promote_int(S, T)
+    return Int64 if R is Bool else R
 
 
 def promote(a, b):
@@ -137,4 +178,6 @@ def promote(a, b):
         return a
     if b.kind == "float":
         return b
-    return Int64
+    if a.kind == "fixed" or b.kind == "fixed":
+        return Float32
+    return promote_int(a, b)

@@ -13,9 +13,9 @@ from . import _lib
 from .border import (STYLES, AbstractBorder, Fill, Inner, NA, NoPad, Pad, borderinstance, calculate_padding,
                      convert_fill)
 from .device import DeviceArray, empty_like, host_dense, ifm_array
-from .eltypes import (BFloat16, Bool, Float16, Float32, Float64, Int64, N0f8, UInt8, as_eltype, eltype_of,
-                      is_eltype_spec, mul_type, promote)
-from .errors import ArgumentError, DimensionMismatch, MethodError
+from .eltypes import (INT_INFO, BFloat16, Bool, Float16, Float32, Float64, Int8, Int64, N0f8, UInt8, as_eltype,
+                      eltype_of, is_eltype_spec, mul_type, promote, signed_type)
+from .errors import ArgumentError, DimensionMismatch, InexactError, MethodError
 from .kernel import Laplacian
 from .kernelfactors import ReshapedOneD, TriggsSdika
 from .offsetarrays import OffsetArray, kernelshift
@@ -128,8 +128,17 @@ def filter_type(S, kernel):
     ks = kernel if isinstance(kernel, tuple) else (kernel,)
     T = None
     for k in ks:
-        if isinstance(k, Laplacian):
-            t = S if S.kind == "float" else (Float32 if S is N0f8 else Int64)
+        if isinstance(k, Laplacian):          # src/imfilter.jl:1136-1147
+            if S.kind == "float":
+                t = S
+            elif S is N0f8:
+                t = Float32
+            elif S is Bool:
+                t = Int8
+            elif INT_INFO[S.name][1]:
+                t = S
+            else:
+                t = signed_type(S)
         else:
             t = mul_type(S, _kernel_dtype(k))
         T = t if T is None else promote(T, t)
@@ -145,7 +154,12 @@ class _Comp:
 
 def _components(kernel, N):
     comps = []
-    for k in kernel:
+    # a tuple of bare TriggsSdika filters dimension d with kernel[d]
+    # (_imfilter_inplace_tuple!, src/imfilter.jl:891-945)
+    bare_iir = bool(kernel) and all(isinstance(k, TriggsSdika) for k in kernel)
+    if bare_iir and len(kernel) > N:
+        raise DimensionMismatch("cannot have more kernels than dimensions")
+    for pos, k in enumerate(kernel):
         if isinstance(k, Laplacian):
             if len(k.flags) != N:
                 raise DimensionMismatch("Laplacian dimensionality must match the image")
@@ -155,7 +169,7 @@ def _components(kernel, N):
             if N < 1:
                 raise DimensionMismatch("kernel dimensionality")
             if not k.iscopy():
-                comps.append(_Comp("iir", axis=0, iir=k))
+                comps.append(_Comp("iir", axis=pos if bare_iir else 0, iir=k))
             continue
         if isinstance(k, ReshapedOneD):
             if k.N > N:
@@ -515,12 +529,14 @@ def imfilter(*args):
     if isinstance(kernel, TriggsSdika):
         kernel = (kernel,)
     kernel = _processed(kernel)
+    if isinstance(img, ColorImage):
+        return _imfilter_color(r, T, img, kernel, border, alg)
     img = img if isinstance(img, DeviceArray) else host_dense(img)
     S = eltype_of(img)
     if T is None:
         T = filter_type(S, kernel)
-    if T.code is None or T in (UInt8, N0f8):
-        raise NotImplementedError("integer output eltype %s: pass a floating-point T" % T)
+    if T.kind != "float":
+        return _imfilter_int(r, T, img, kernel, border, alg)
     img = _library_input(img)
     border = borderinstance(border)
     N = len(img.shape)
@@ -570,6 +586,15 @@ def imfilter_(*args):
     if len(args) < 3:
         raise MethodError("imfilter!([r], out, img, kernel, ...)")
     out, img, kernel, rest = args[0], args[1], args[2], args[3:]
+    T_out = eltype_of(_outdata(out))
+    if T_out.kind != "float" and not (isinstance(kernel, TriggsSdika) and rest and isinstance(rest[0], (int, np.integer))):
+        # an integer out: filter into a Float64 block of its layout, then convert
+        tmp = empty_like(_outdata(out), Float64)
+        targs = ([r] if r is not None else []) + [OffsetArray(tmp, out.first) if isinstance(out, OffsetArray) else tmp,
+                                                  img, kernel] + list(rest)
+        imfilter_(*targs)
+        _to_int(tmp, T_out, _outdata(out))
+        return out
     if isinstance(kernel, TriggsSdika) and rest and isinstance(rest[0], (int, np.integer)):
         dim = int(rest[0])                                  # 1-based like the reference
         border = rest[1] if len(rest) > 1 else "replicate"
@@ -604,6 +629,141 @@ def imfilter_(*args):
         raise DimensionMismatch("output axes %r disagree with the image axes %r" % (np.shape(out), img.shape))
     _run(_choose(r, alg, img, kernel), _outdata(out), img, kernel, border, None)
     return out
+
+
+def _integer_taps(kernel):
+    for k in kernel:
+        if isinstance(k, Laplacian):
+            continue
+        if isinstance(k, TriggsSdika) or (isinstance(k, ReshapedOneD) and k.is_iir):
+            return None
+        d = np.asarray(k.data.data if isinstance(k, ReshapedOneD) else (k.data if isinstance(k, OffsetArray) else k),
+                       dtype=np.float64)
+        if not np.all(d == np.round(d)):
+            return None
+    return True
+
+
+def _tap_l1(k):
+    if isinstance(k, Laplacian):
+        return 4.0 * sum(k.flags)
+    d = k.data.data if isinstance(k, ReshapedOneD) else (k.data if isinstance(k, OffsetArray) else k)
+    return float(np.sum(np.abs(np.asarray(d, dtype=np.float64))))
+
+
+def _imfilter_int(r, T, img, kernel, border, alg):
+    """An integer output eltype (filter_type of integer images and taps, or an
+    explicit T): the filter runs in floating point on the GPU and the results
+    are converted on the device, InexactError where a value is not an integer
+    of T's range (src/imfilter.jl:233-254).  With integer taps the FP32 sums
+    are exact while every partial sum stays below 2^24; calls that could pass
+    it are refused rather than rounded."""
+    if T is Bool:
+        raise NotImplementedError("Bool output eltype")
+    S = eltype_of(img)
+    if S.kind in ("int", "fixed") and _integer_taps(kernel):
+        if isinstance(img, DeviceArray) or S is N0f8:
+            amax = 255.0 if S in (UInt8, N0f8) else float(2 ** (INT_INFO[S.name][0] - (1 if INT_INFO[S.name][1] else 0)))
+        else:
+            a = np.asarray(img)
+            amax = float(np.max(np.abs(a.astype(np.float64)))) if a.size else 0.0
+        bound = amax
+        for k in kernel:
+            bound *= _tap_l1(k)
+        if bound >= 2.0 ** 24:
+            raise NotImplementedError("integer result with partial sums up to %g: exact only below 2^24" % bound)
+    args = ([r] if r is not None else []) + [Float64, img, kernel, border] + ([alg] if alg is not None else [])
+    res = imfilter(*args)
+    data = res.data if isinstance(res, OffsetArray) else res
+    out = _to_int(data, T)
+    return OffsetArray(out, res.first) if isinstance(res, OffsetArray) else out
+
+
+def _to_int(data, T, out=None):
+    import ctypes
+    if T is N0f8:
+        bits, signed, scale = 8, False, 255.0
+    else:
+        bits, signed = INT_INFO[T.name]
+        scale = 1.0
+    if out is None:
+        out = empty_like(data, T)
+    inexact = ctypes.c_int(0)
+    if isinstance(out, DeviceArray):
+        dst, dev = out.ptr, 1
+    else:
+        dst, dev = np.asarray(out).ctypes.data, 0
+    _lib.check(_lib.load().ifm_convert_int(_lib.ctx(), ifm_array(data), dst, dev, bits, 1 if signed else 0, scale,
+                                           ctypes.byref(inexact)))
+    if inexact.value:
+        raise InexactError("convert(%s, x): a filtered value is not an integer of %s's range "
+                           "(consider imfilter(Float64, img, kernel, ...))" % (T, T))
+    return out
+
+
+class ColorImage:
+    """A colour image in channelview form: `channels` holds the colour
+    channels along a leading axis (RGB: 3), the image axes after it.  Every
+    channel is filtered with the same kernel, as imfilter does for Colorant
+    arrays (arithmetic per channel; filtfft's channelview, src/imfilter.jl:868-883)."""
+
+    def __init__(self, channels, colorant="RGB"):
+        self.channels = channels
+        self.colorant = colorant
+
+    @property
+    def shape(self):
+        return tuple(self.channels.shape[1:])
+
+    @property
+    def ndim(self):
+        return len(self.shape)
+
+    def __repr__(self):
+        return "ColorImage(%s, %s)" % (self.colorant, self.shape)
+
+
+def colorview(colorant, channels):
+    """colorview(RGB, A): the channel-first array A as a colour image."""
+    return ColorImage(channels, colorant)
+
+
+def channelview(img):
+    return img.channels
+
+
+def _prepend_axis(kernel, N):
+    out = []
+    bare = all(isinstance(k, TriggsSdika) for k in kernel)
+    for pos, k in enumerate(kernel):
+        if isinstance(k, Laplacian):
+            out.append(Laplacian((False,) + tuple(k.flags)))
+        elif isinstance(k, ReshapedOneD):
+            out.append(ReshapedOneD(k.N + 1, k.Npre + 1, k.data))
+        elif isinstance(k, TriggsSdika):
+            out.append(ReshapedOneD(N + 1, 1 + (pos if bare else 0), k))
+        else:
+            a = _extend(_as_offset(k), N)
+            out.append(OffsetArray(np.asarray(a.data).reshape((1,) + a.shape), (0,) + tuple(a.first)))
+    return tuple(out)
+
+
+def _imfilter_color(r, T, img, kernel, border, alg):
+    N = len(img.shape)
+    k2 = _prepend_axis(kernel, N)
+    b = borderinstance(border)
+    if isinstance(b, (Pad, Fill, Inner)) and getattr(b, "lo", None) is not None:
+        lo, hi = (0,) + tuple(b.lo), (0,) + tuple(b.hi)
+        b = Fill(b.value, lo, hi) if isinstance(b, Fill) else (Pad(b.style, lo, hi) if isinstance(b, Pad)
+                                                              else Inner(lo, hi))
+    if isinstance(b, NA):
+        raise NotImplementedError("NA() on colour images")
+    args = ([r] if r is not None else []) + ([T] if T is not None else []) + [img.channels, k2, b] + \
+        ([alg] if alg is not None else [])
+    res = imfilter(*args)
+    if isinstance(res, OffsetArray):
+        return OffsetArray(ColorImage(res.data, img.colorant), res.first[1:])
+    return ColorImage(res, img.colorant)
 
 
 def _library_input(img):

@@ -19,8 +19,9 @@
  *   - Every entry point returns an ifm_status; ifm_last_error gives the message.
  *     IFM_E_ARG maps to Julia's ArgumentError and IFM_E_DIM to DimensionMismatch.
  *   - Border Fill values arrive already converted to the input eltype
- *     (src/borderarray.jl:11-24); they are passed as a double, which holds
- *     every value of every supported eltype exactly.
+ *     (src/borderarray.jl:11-24) and are passed as that value in a double:
+ *     exact for every eltype but N0f8, whose value code/255 is passed (the
+ *     library recovers the code as round(255 * fill)).
  */
 #ifndef IFM_H
 #define IFM_H
@@ -199,6 +200,16 @@ int ifm_divide(ifm_ctx *ctx, ifm_array *num, const ifm_array *den);
  * - 1 of dst (src/imfilter.jl:372-375, :433-434): the NoPad sub-range form of
  * imfilter! and the IIR stages of mixed cascades.  Eltypes may differ (convert). */
 int ifm_copy_into(ifm_ctx *ctx, const ifm_array *src, ifm_array *dst, const int64_t *dst_first);
+
+/* convert(T, x) into an integer output eltype T (bits 8/16/32/64, signed or
+ * not) of floating-point filter results, each first multiplied by scale (1,
+ * or 255 for N0f8 codes, which round to the nearest code): the store into an
+ * integer out array of __imfilter_inbounds! (src/imfilter.jl:650-669).  dst
+ * has src's dense layout (device memory when dst_device, else host).
+ * *inexact = 1 when a value is not an integer (scale 1) or lies outside T's
+ * range; Julia's convert throws InexactError there (src/imfilter.jl:233-254). */
+int ifm_convert_int(ifm_ctx *ctx, const ifm_array *src, void *dst, int dst_device, int bits, int is_signed,
+                    double scale, int *inexact);
 
 /* findlocalmaxima (sense = +1) / findlocalminima (sense = -1)
  * (src/extrema.jl:122-163): flags[i] = 1 where img[i] is strictly greater

@@ -117,8 +117,12 @@ def _region(a, ofirst, osize):
 
 
 def imfilter_dense(img, k, kfirst, border, ofirst=None, osize=None):
-    """Dense correlation of img with kernel k whose axis d starts at kfirst[d]."""
+    """Dense correlation of img with kernel k whose axis d starts at kfirst[d]
+    (a kernel with fewer dimensions extends with unit axes at offset 0)."""
     a, ap = _f64(img)
+    k = np.asarray(k, dtype=np.float64)
+    k = k.reshape(k.shape + (1,) * (a.ndim - k.ndim))
+    kfirst = list(kfirst) + [0] * (a.ndim - len(kfirst))
     kk, kp = _f64(k)
     s, fill = _style(border)
     of, os_ = _region(a, ofirst, osize)

@@ -453,3 +453,11 @@ def test_mixed_cascade_replicate_equals_long_constant_extension():
     x = O.imfilter_iir(ext, [stages[0][1:]], "replicate")
     x = O.imfilter_factors(x, [(0, np.array(stages[1][2]), -1), (0, np.array(stages[2][2]), 0)], "replicate")
     np.testing.assert_allclose(got, x[400:440], rtol=1e-9, atol=1e-12)
+
+
+def test_dense_kernel_with_fewer_dims_extends():
+    rng = np.random.default_rng(64)
+    img = rng.random((12, 9))
+    k = rng.standard_normal(3)
+    np.testing.assert_array_equal(O.imfilter_dense(img, k, [-1], "reflect"),
+                                  O.imfilter_dense(img, k.reshape(3, 1), [-1, 0], "reflect"))

@@ -879,7 +879,7 @@ int ifm_imfilter_dense(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const 
   bool mfma = false;
   if (precision == IFM_PREC_F16X3) {
     mfma = twod && dense_mfma_supported(dk, vi.d, vo.d);
-    if (!mfma) return fail(ctx, IFM_E_UNSUPPORTED, "matrix-core path needs a 2-D kernel of at most 49 x 32 taps");
+    if (!mfma) return fail(ctx, IFM_E_UNSUPPORTED, "matrix-core path needs a 2-D kernel of at most 49 x 32 taps below 2^100 in magnitude");
   } else if (precision == IFM_PREC_AUTO)
     mfma = twod && dk.n[0] >= 15 && dk.n[1] >= 15 && dense_mfma_supported(dk, vi.d, vo.d);
   else if (precision != IFM_PREC_FP32) return fail(ctx, IFM_E_ARG, "unknown precision");

@@ -370,7 +370,7 @@ int build_batch(ifm_ctx *ctx, const ifm_request *reqs, const int *idx, int n, co
       if (q.op != IFM_OP_DENSE) continue;
       dks[i] = dense_of(q.kernel);
       if (!dense_mfma_supported(dks[i], vin[i], vout[i]))
-        return sfail(ctx, IFM_E_UNSUPPORTED, "dense requests need a 2-D kernel of at most 49 x 32 taps");
+        return sfail(ctx, IFM_E_UNSUPPORTED, "dense requests need a 2-D kernel of at most 49 x 32 taps below 2^100 in magnitude");
       byvar[dense_mfma_variant(dks[i])].push_back(i);
     }
     // Toeplitz tables are built on the device by one launch ahead of the dense ones.
@@ -587,11 +587,14 @@ int ifm_serve(ifm_ctx *ctx, const ifm_request *reqs, const double *arrival_s, in
   std::vector<ifm_request> rq(reqs, reqs + n);
   std::vector<uint64_t> need(n), offs(3 * (size_t)n, 0);
   bool any_gen = false;
+  // need[i]: the arena bytes request i holds while in flight, each array
+  // rounded up to the arena's 256-byte granule as Arena::alloc rounds it.
+  auto granule = [](uint64_t b) { return (b + 255) & ~(uint64_t)255; };
   for (int i = 0; i < n; ++i) {
-    need[i] = array_bytes(rq[i].in) + array_bytes(rq[i].out) +
-              ((rq[i].op == IFM_OP_SEPARABLE && rq[i].nfactors2 > 0) ? array_bytes(rq[i].out2) : 0);
+    need[i] = granule(array_bytes(rq[i].in)) + granule(array_bytes(rq[i].out)) +
+              ((rq[i].op == IFM_OP_SEPARABLE && rq[i].nfactors2 > 0) ? granule(array_bytes(rq[i].out2)) : 0);
     if (!rq[i].in.data) any_gen = true;
-    if (need[i] + 512 > cfg->budget_bytes) return sfail(ctx, IFM_E_BUDGET, "a request exceeds the HBM budget");
+    if (need[i] > cfg->budget_bytes) return sfail(ctx, IFM_E_BUDGET, "a request exceeds the HBM budget");
   }
   Arena arena;
   char *abase = nullptr;
@@ -702,6 +705,12 @@ int ifm_serve(ifm_ctx *ctx, const ifm_request *reqs, const double *arrival_s, in
           gen = gen || !reqs[next].in.data;
           inflight_bytes += need[next];
           members.push_back(next++);
+        }
+        // With nothing in flight the arena is whole, so a request that fits
+        // the budget always gets its buffers; report rather than spin if not.
+        if (members.empty() && flights.empty() && next < n && arrival_s[next] <= now()) {
+          rc = sfail(ctx, IFM_E_BUDGET, "a request does not fit the empty arena");
+          break;
         }
         if (!members.empty()) {
           Block blk;

@@ -590,24 +590,33 @@ __global__ void __launch_bounds__(256, 2) k_dense_mfma(MfmaArgs a0, MfBatch bt) 
     __syncthreads();
     amax = fmaxf(fmaxf(red[par][0], red[par][1]), fmaxf(red[par][2], red[par][3]));
     const bool tilebad = redflag[par][0] | redflag[par][1] | redflag[par][2] | redflag[par][3];
-    // A tile holding Inf or NaN is summed directly; it then follows the same
-    // path as the others with its matrix work skipped and its stores dropped.
-    if (tilebad) mf_direct(a, T);
     // Power-of-two scale putting the tile's largest |x| in [0.5, 1).
     int e = 0;
     if (amax > 0.f) frexpf(amax, &e);
-    e = e < -100 ? -100 : (e > 100 ? 100 : e);
+    // A tile holding Inf or NaN, or one whose largest |x| lies outside
+    // [2^-100, 2^100) (its scaled hi / lo halves would leave FP16's range or
+    // its scale factors FP32's), or whose output scale 2^e * kinv would not
+    // be a normal FP32 number, is summed directly in FP32; it then follows
+    // the same path as the others with its matrix work skipped and its
+    // stores dropped.  The decision is uniform over the workgroup.
+    int ke = 0;
+    frexpf(a.kinv, &ke);  // kinv = 2^(ke - 1)
+    const bool direct = tilebad || e > 100 || e < -100 || e + ke > 120 || e + ke < -120;
+    if (direct) {
+      mf_direct(a, T);
+      e = 0;
+    }
     const float xs = ldexpf(1.0f, -e);
     const float xinv = ldexpf(1.0f, e);
     if constexpr (G::NB == 1) {
-      if (!tilebad) mf_put<G, 0>(lds32, raw, xs, true);
+      if (!direct) mf_put<G, 0>(lds32, raw, xs, true);
       if (t + 1 < t1) {
         int ln, rn;
         const MfmaArgs &an = mf_args<BATCH>(a0, bt, t + 1, ln, rn);
         mf_maps<G>(an, mf_tile(an, ln), rmap, cmap);
       }
     } else {
-      if (!tilebad) mf_convert<G>(lds32, xs);
+      if (!direct) mf_convert<G>(lds32, xs);
     }
     __syncthreads();
     if constexpr (G::NB == 1) {
@@ -629,7 +638,7 @@ __global__ void __launch_bounds__(256, 2) k_dense_mfma(MfmaArgs a0, MfBatch bt) 
     f32x4 acc[MF_B];
 #pragma unroll
     for (int b = 0; b < MF_B; ++b) acc[b] = f32x4{0.f, 0.f, 0.f, 0.f};
-    if (!tilebad) {
+    if (!direct) {
       // chunk c = kb * NCH + ch covers K block kb and kernel rows 8ch .. 8ch+7
       if constexpr (NC == 1) {
         mf_chunk<G>(lw, ahi, alo, acc);
@@ -648,7 +657,7 @@ __global__ void __launch_bounds__(256, 2) k_dense_mfma(MfmaArgs a0, MfBatch bt) 
         }
       }
     }
-    mf_store(a, T, acc, xinv * a.kinv, !tilebad);
+    mf_store(a, T, acc, xinv * a.kinv, !direct);
   }
 }
 
@@ -698,6 +707,15 @@ static int device_attr(hipDeviceAttribute_t attr, int dflt) {
 
 bool dense_mfma_supported(const DenseKernel &k, const Dense4 &in, const Dense4 &out) {
   const bool twod = k.n[2] == 1 && k.n[3] == 1;
+  // non-finite taps, or taps whose largest |k| lies outside [2^-100, 2^100):
+  // the scaled FP16 halves would not hold them (the FP32 VALU path takes
+  // such kernels); an all-zero kernel is fine
+  float kmax = 0.f;
+  for (float t : k.taps) {
+    if (!std::isfinite(t)) return false;
+    kmax = fmaxf(kmax, fabsf(t));
+  }
+  if (kmax != 0.f && !(kmax >= ldexpf(1.f, -100) && kmax < ldexpf(1.f, 100))) return false;
   if (!(twod && mfma_kb(k.n[0]) <= MF_MAXKB && k.n[1] <= MF_MAXCH * MF_UCH)) return false;
   // buffer descriptors address one plane with 31-bit byte offsets
   if ((double)in.n[0] * (double)in.n[1] * (double)dtype_size(in.dtype) >= 2147483648.0) return false;

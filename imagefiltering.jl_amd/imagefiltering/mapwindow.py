@@ -176,9 +176,12 @@ def _wrap(res, first, border, indices):
 
 
 def _pair(omin, omax):
+    """(min, max) pairs as a trailing axis of length 2, in the eltype of the
+    results (N0f8 and BFloat16 keep their tags)."""
     if isinstance(omin, DeviceArray):
         return (omin, omax)
-    return np.stack([np.asarray(omin), np.asarray(omax)], axis=-1)
+    out = np.stack([np.asarray(omin), np.asarray(omax)], axis=-1)
+    return out.view(type(omin)) if type(omin) is not np.ndarray else out
 
 
 def _extrema_filter(img, window):

@@ -139,6 +139,18 @@ def request_bytes(mix):
     return px * (ein + nout * eout)
 
 
+def arena_bytes(mix):
+    """Arena bytes a generated request holds in ``serve``: its input and
+    outputs, each rounded up to the arena's 256-byte granule."""
+    px = request_pixels(mix)
+    ein = np.where(mix["dtype"] == 1, 2, 4)
+    op = mix["op"]
+    eout = np.where(op == OPS.index("median7"), ein, 8)
+    nout = np.where(op == OPS.index("sobel"), 2, 1)
+    g = lambda b: (b + 255) // 256 * 256
+    return g(px * ein) + nout * g(px * eout)
+
+
 def serve(mix, mode="batched", budget_bytes=64 << 30, max_batch=1024, max_inflight=4, buffers=None):
     """Run a mix through ifm_serve; returns per-request admit and done times (s)."""
     L, ctx = _lib.load(), _lib.ctx()

@@ -60,7 +60,8 @@ typedef enum {
   IFM_PREC_AUTO = 0,    /* library choice: matrix cores for 2-D kernels >= 15x15, else FP32 VALU */
   IFM_PREC_FP32 = 1,    /* FP32 FMA on the vector ALUs */
   IFM_PREC_F16X3 = 2    /* matrix cores: scaled x and k split in FP16 hi+lo, 3 products, FP32
-                           accumulate (2-D kernels up to 49 x 32; error ~2^-22 relative) */
+                           accumulate (2-D kernels up to 49 x 32 of finite taps below 2^100;
+                           error ~2^-22 relative) */
 } ifm_precision;
 
 typedef struct {

@@ -378,6 +378,81 @@ def test_dense_mfma_nonfinite_tiles():
     assert relerr(np.where(fin, out, 0), np.where(fin, want, 0)) <= TOL
 
 
+def _abs_bound(img, k, first, border):
+    """sum |k| |x| over each output's window: the scale the FP32 error of one
+    output is measured against."""
+    return O.imfilter_dense(np.abs(img.astype(np.float64)), np.abs(k), first, border)
+
+
+@pytest.mark.parametrize("kscale,scales", [
+    (1.0, [1e36, 1e-36, 3e-30, 1.0]),      # tile exponents beyond +-100 and inside
+    (1e25, [1e9, 1e11]),                   # output scale 2^(e + ke) near and past 2^120
+    (1e-25, [1e-10, 1e-12]),               # and near and past 2^-120
+])
+def test_dense_mfma_extreme_magnitudes(kscale, scales):
+    """Images whose tiles span FP32's range.  Each 64 x 128 output tile takes
+    the FP16 hi / lo split at its own power-of-two scale, or the direct FP32
+    sum where that scale or the output's would leave the formats' range.
+    An image of one magnitude keeps every output within 1e-5 of sum |k| |x|
+    of the FP64 result; where magnitudes mix, an output is within 1e-5 of
+    sum |k| times the largest |x| its tile reads (values more than 2^24 below
+    that are beyond the split's reach), and nothing overflows."""
+    from scipy.ndimage import maximum_filter
+    from imagefiltering import _lib
+    rng = np.random.default_rng(71)
+    k = norm_random_kernel(rng, 15) * kscale
+    first = [-7, -7]
+    base = rng.random((512, 448)) + 0.5
+    blocks = np.array(scales)[(np.arange(512)[:, None] // 128 + np.arange(448)[None, :] // 64) % len(scales)]
+    img = np.asfortranarray((base * blocks).astype(np.float32))
+    # a box around each output holding every input its tile reads (a tile
+    # spans 64 outputs along axis 0 and 128 along axis 1, plus the 14-tap halo)
+    tilemax = maximum_filter(np.abs(img.astype(np.float64)), size=(2 * 78 + 1, 2 * 142 + 1), mode="nearest")
+    for b in ["replicate", IF.Fill(0.0)]:
+        out = _dense_lib(img, k, first, b if isinstance(b, IF.Fill) else IF.Pad(b), _lib.PREC_F16X3)
+        want = O.imfilter_dense(img.astype(np.float64), k, first, obord(b))
+        assert np.all(np.isfinite(out)), (kscale, b)
+        err = np.abs(out.astype(np.float64) - want) / (np.abs(k).sum() * tilemax)
+        assert err.max() <= TOL, (kscale, b, float(err.max()))
+    # one image per magnitude, through the explicit split and the default dispatch
+    for s in scales:
+        x = np.asfortranarray((base * s).astype(np.float32))
+        out = _dense_lib(x, k, first, IF.Pad("symmetric"), _lib.PREC_F16X3)
+        want = O.imfilter_dense(x.astype(np.float64), k, first, "symmetric")
+        err = np.abs(out.astype(np.float64) - want) / _abs_bound(x, k, first, "symmetric")
+        assert err.max() <= TOL, (kscale, s, float(err.max()))
+    # The drop-in call first runs the reference's rank test, whose threshold
+    # sqrt(eps) is absolute (src/imfilter.jl:1167-1174): taps of ~1e-25 have
+    # every singular value below it, so the kernel applied is the rank-1
+    # factor pair, as in the reference.
+    from imagefiltering.imfilter import factorkernel
+    fac = factorkernel(IF.OffsetArray(k, *first))
+    assert len(fac) == (2 if kscale < 1e-10 else 1)
+    kd = np.outer(fac[0].data.ravel(), fac[1].data.ravel()) if len(fac) == 2 else k
+    for s in scales:
+        x = np.asfortranarray((base * s).astype(np.float32))
+        out = IF.imfilter(x, IF.OffsetArray(k, *first), "reflect", IF.Algorithm.FIR())
+        want = O.imfilter_dense(x.astype(np.float64), kd, first, "reflect")
+        err = np.abs(out.astype(np.float64) - want) / _abs_bound(x, kd, first, "reflect")
+        assert err.max() <= TOL, (kscale, s, float(err.max()))
+
+
+def test_dense_mfma_kernel_beyond_split_range():
+    """Taps of magnitude 2^100 and beyond are outside the FP16 split's range:
+    the explicit matrix-core precision refuses them and the default dispatch
+    sums them in FP32."""
+    from imagefiltering import _lib
+    rng = np.random.default_rng(72)
+    k = norm_random_kernel(rng, 15) * 1e31
+    img = np.asfortranarray(rng.random((200, 180)).astype(np.float32))
+    with pytest.raises(NotImplementedError):
+        _dense_lib(img, k, [-7, -7], IF.Pad("replicate"), _lib.PREC_F16X3)
+    out = IF.imfilter(img, IF.OffsetArray(k, -7, -7), "replicate", IF.Algorithm.FIR())
+    want = O.imfilter_dense(img.astype(np.float64), k, [-7, -7], "replicate")
+    err = np.abs(out.astype(np.float64) - want) / _abs_bound(img, k, [-7, -7], "replicate")
+    assert err.max() <= TOL
+
+
 def test_dense_mfma_bf16_storage_and_log():
     rng = np.random.default_rng(18)
     x = rng.random((200, 190)).astype(np.float32)
@@ -763,6 +838,92 @@ def test_extrema_lemire_semantics():
         got = IF.mapwindow(IF.extrema, A, window)
         np.testing.assert_array_equal(got[..., 0], mn.astype(np.float32))
         np.testing.assert_array_equal(got[..., 1], mx.astype(np.float32))
+
+
+@pytest.mark.parametrize("dtype", ["u8", "n0f8"])
+def test_extrema_integer_types(dtype):
+    """extrema_filter on UInt8 and N0f8 (the order-filter benchmark form):
+    the reference's 1-D vectors as codes, and 2-D / 3-D windows, odd and
+    even, against the Lemire oracle; results stay in the input eltype."""
+    import json, os
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_fixtures.json")))
+    wrap = IF.n0f8 if dtype == "n0f8" else (lambda c: c)
+    for c in g["extrema_1d"]["cases"]:
+        # the vectors' values in tenths, as codes (x -> 50 x keeps their order)
+        x = np.rint(np.asarray(c["input"]) * 50).astype(np.uint8)
+        want = np.rint(np.asarray(c["expect"]) * 50).astype(np.uint8)
+        got = IF.mapwindow(IF.extrema, wrap(x), c["window"])
+        np.testing.assert_array_equal(np.asarray(got).view(np.uint8), want, err_msg=str(c))
+    rng = np.random.default_rng(26)
+    codes = rng.integers(0, 256, (301, 170)).astype(np.uint8)
+    codes[rng.random(codes.shape) < 0.05] = 255
+    for window in [(3, 3), (9, 9), (2, 5), (8, 3), (1, 17)]:
+        mn, mx = O.extrema_lemire(codes, list(window))
+        got = IF.mapwindow(IF.extrema, wrap(codes), window)
+        if dtype == "n0f8":
+            assert isinstance(got, IF.N0f8Array)
+        else:
+            assert got.dtype == np.uint8
+        np.testing.assert_array_equal(np.asarray(got[..., 0]).view(np.uint8), mn.astype(np.uint8), str(window))
+        np.testing.assert_array_equal(np.asarray(got[..., 1]).view(np.uint8), mx.astype(np.uint8), str(window))
+    c3 = rng.integers(0, 256, (40, 33, 9)).astype(np.uint8)
+    mn, mx = O.extrema_lemire(c3, [3, 4, 3])
+    got = IF.mapwindow(IF.extrema, wrap(c3), (3, 4, 3))
+    np.testing.assert_array_equal(np.asarray(got[..., 0]).view(np.uint8), mn.astype(np.uint8))
+    np.testing.assert_array_equal(np.asarray(got[..., 1]).view(np.uint8), mx.astype(np.uint8))
+
+
+def test_cascade_fill_non_unit_factors():
+    """Cascades that run as separate passes (an axis-2 factor too long for the
+    fused 3-D pass, an identity factor on axis 1, UInt8 input) with
+    Fill(0.7) (Fill(7) for UInt8) and sobel factors whose sums are not 1 (0 for the derivative),
+    so the fill value each later pass sees is the filtered fill, not 0.7."""
+    rng = np.random.default_rng(43)
+    vol = np.asfortranarray(rng.random((40, 31, 37)).astype(np.float32))
+    zt = np.linspace(0.1, 0.3, 11)                                  # sum 2.2, 11 taps
+    long_z = IF.OffsetArray(zt.reshape(1, 1, 11), 0, 0, -4)
+    cases = []
+    for d in range(3):
+        sob = KernelFactors.sobel((True, True, True), d)
+        cases.append((tuple(sob) + (long_z,), factors_of(sob) + [(2, zt, -4)]))
+    for d in (0, 2):   # an identity factor on axis 1
+        sob = KernelFactors.sobel((True, False, True), d)
+        cases.append((sob, factors_of(sob)))
+    for kern, fs in cases:
+        for b in [IF.Fill(0.7), "reflect"]:
+            out = IF.imfilter(vol, kern, b)
+            want = O.imfilter_factors(vol.astype(np.float64), fs, obord(b))
+            assert relerr(out, want) <= TOL, (len(kern), b)
+    codes = np.asfortranarray(rng.integers(0, 256, (30, 26, 12)).astype(np.uint8))
+    kern = KernelFactors.sobel((True, True, True), 1)
+    out = IF.imfilter(codes, kern, IF.Fill(7))
+    want = O.imfilter_factors(codes.astype(np.float64), factors_of(kern), ("fill", 7.0))
+    assert relerr(out, want) <= TOL
+    # the fill value converts to the image's eltype first (src/borderarray.jl:11-20)
+    with pytest.raises(IF.ArgumentError):
+        IF.imfilter(codes, kern, IF.Fill(0.7))
+
+
+@pytest.mark.parametrize("border", STYLES + [IF.Fill(0.0), IF.Fill(0.7)])
+def test_cascade_reference_identities(border):
+    """test/cascade.jl:4-39: a tuple of factors equals the dense kernel of
+    their convolution, for repeated factors along one axis and factors in
+    any order, every border (here also Fill(0.7))."""
+    rng = np.random.default_rng(44)
+    a1 = rng.random(15)
+    kern = IF.OffsetArray(np.ones(3), -1)
+    kern2 = IF.OffsetArray(np.array([1.0, 2, 3, 2, 1]), -2)
+    np.testing.assert_allclose(IF.imfilter(a1, (kern, kern), border), IF.imfilter(a1, kern2, border),
+                               rtol=1e-6, atol=1e-6)
+    a = rng.random((15, 15))
+    kx, ky = IF.OffsetArray(np.ones((3, 1)), -1, 0), IF.OffsetArray(np.ones((1, 3)), 0, -1)
+    c = np.array([1.0, 2, 3, 2, 1])
+    k2 = IF.OffsetArray(np.outer(c, c), -2, -2)
+    k2x = IF.OffsetArray(np.outer(c, np.ones(3)), -2, -1)
+    k2y = IF.OffsetArray(np.outer(np.ones(3), c), -1, -2)
+    for fac, dense in [((kx, ky, kx, ky), k2), ((kx, kx, ky, ky), k2), ((kx, kx, ky), k2x), ((ky, kx, ky), k2y)]:
+        np.testing.assert_allclose(IF.imfilter(a, fac, border), IF.imfilter(a, dense, border), rtol=1e-6,
+                                   atol=1e-5, err_msg=str(border))
 
 
 def test_errors_from_the_library():

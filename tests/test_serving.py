@@ -157,8 +157,8 @@ def test_batched_equals_alone_and_percall_and_oracle():
 def test_serve_generated_inputs_under_a_small_budget():
     mix = serving.request_mix(60, rate=5000.0, seed=5, side=(64, 700))
     budget = 24 << 20
-    need = serving.request_bytes(mix)
-    keep = need <= budget - 4096
+    need = serving.arena_bytes(mix)
+    keep = need <= budget
     for k in ("w", "h", "dtype", "op", "border", "sigma", "kseed", "arrival"):
         mix[k] = mix[k][keep]
     mix["n"] = int(keep.sum())
@@ -168,6 +168,24 @@ def test_serve_generated_inputs_under_a_small_budget():
         assert np.all(done >= admit)
         st = serving.stats(mix, done)
         assert st["images_per_s"] > 0
+
+
+@pytest.mark.gpu
+def test_serve_request_filling_the_whole_budget():
+    """A two-output request (sobel) whose 256-byte-rounded arrays fill the
+    budget exactly is served in both modes; one granule less is refused with
+    a budget error instead of waiting forever."""
+    mix = serving.request_mix(3, rate=None, seed=9, side=(97, 131))
+    mix["op"][:] = serving.OPS.index("sobel")
+    need = serving.arena_bytes(mix)
+    budget = int(need.max())
+    assert budget > serving.request_bytes(mix).max()   # the rounding matters here
+    for mode in ("batched", "percall"):
+        admit, done = serving.serve(mix, mode, budget_bytes=budget, max_batch=4)
+        assert np.all(done >= admit) and np.all(done > 0)
+    for mode in ("batched", "percall"):
+        with pytest.raises(RuntimeError, match="budget"):
+            serving.serve(mix, mode, budget_bytes=budget - 256, max_batch=4)
 
 
 @pytest.mark.gpu

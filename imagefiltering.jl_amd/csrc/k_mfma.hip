@@ -311,6 +311,22 @@ __device__ __forceinline__ void mf_tload(const uint16_t *ttab, int c, f16x8 (&th
   }
 }
 
+// The first nu rows of chunk c (the others are left zero).
+__device__ __forceinline__ void mf_tload_rows(const uint16_t *ttab, int c, int nu, f16x8 (&thi)[MF_UCH],
+                                              f16x8 (&tlo)[MF_UCH]) {
+  const u16x8 *tt = (const u16x8 *)ttab + (size_t)c * (MF_UCH * 2 * 64) + (threadIdx.x & 63);
+#pragma unroll
+  for (int u = 0; u < MF_UCH; ++u) {
+    if (u < nu) {
+      thi[u] = __builtin_bit_cast(f16x8, tt[(2 * u) * 64]);
+      tlo[u] = __builtin_bit_cast(f16x8, tt[(2 * u + 1) * 64]);
+    } else {
+      thi[u] = f16x8{};
+      tlo[u] = f16x8{};
+    }
+  }
+}
+
 // Waits for every outstanding vector-memory operation of the wave.  Used
 // right after the Toeplitz fragments are loaded: the sweep then carries no
 // wait for them, which (the counter being in order) would also wait for the
@@ -374,11 +390,12 @@ __device__ __forceinline__ void mf_chunk(const uint32_t *lbase, const f16x8 (&th
 // r + 8), so fragment rows 0 .. 22 are read once each and every one between 7
 // and 15 feeds all 8 blocks; the pipeline fills and drains once instead of
 // once per chunk.
-template <class G>
+template <class G, int NU>
 __device__ __forceinline__ void mf_rows16(const uint32_t *lbase, const f16x8 (&ahi)[MF_UCH],
                                           const f16x8 (&alo)[MF_UCH], const f16x8 (&bhi)[MF_UCH],
                                           const f16x8 (&blo)[MF_UCH], f32x4 (&acc)[MF_B]) {
-  constexpr int NR = MF_B + 2 * MF_UCH - 1;
+  // NU kernel rows (15 or 16; a 15-row kernel skips the zero 16th row)
+  constexpr int NR = MF_B + NU - 1;
   const u32x4 *src0 = (const u32x4 *)lbase;
   u32x4 w0 = src0[0], w1 = src0[1];
 #pragma unroll
@@ -403,19 +420,19 @@ __device__ __forceinline__ void mf_rows16(const uint32_t *lbase, const f16x8 (&a
 #pragma unroll
     for (int b = 0; b < MF_B; ++b) {
       const int u = r - b;
-      if (u >= 0 && u < 2 * MF_UCH)
+      if (u >= 0 && u < NU)
         acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fhi, u < MF_UCH ? ahi[u] : bhi[u - MF_UCH], acc[b], 0, 0, 0);
     }
 #pragma unroll
     for (int b = 0; b < MF_B; ++b) {
       const int u = r - b;
-      if (u >= 0 && u < 2 * MF_UCH)
+      if (u >= 0 && u < NU)
         acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(flo, u < MF_UCH ? ahi[u] : bhi[u - MF_UCH], acc[b], 0, 0, 0);
     }
 #pragma unroll
     for (int b = 0; b < MF_B; ++b) {
       const int u = r - b;
-      if (u >= 0 && u < 2 * MF_UCH)
+      if (u >= 0 && u < NU)
         acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fhi, u < MF_UCH ? alo[u] : blo[u - MF_UCH], acc[b], 0, 0, 0);
     }
     w0 = n0;
@@ -521,7 +538,7 @@ __device__ __forceinline__ const MfmaArgs &mf_args(const MfmaArgs &a0, const MfB
   }
 }
 
-template <int KB, int NCH, bool BATCH>
+template <int KB, int NCH, bool BATCH, int NU = MF_UCH * NCH>
 __global__ void __launch_bounds__(256, 2) k_dense_mfma(MfmaArgs a0, MfBatch bt) {
   using G = MfmaGeom<KB, NCH>;
   constexpr int NC = G::NC;
@@ -543,7 +560,7 @@ __global__ void __launch_bounds__(256, 2) k_dense_mfma(MfmaArgs a0, MfBatch bt) 
   const MfmaArgs &af = mf_args<BATCH>(a0, bt, t0, lf, cur);
   f16x8 ahi[MF_UCH], alo[MF_UCH], bhi[MF_UCH], blo[MF_UCH];
   mf_tload(af.ttab, 0, ahi, alo);
-  if constexpr (NC > 1) mf_tload(af.ttab, 1, bhi, blo);
+  if constexpr (NC > 1) mf_tload_rows(af.ttab, 1, NU - MF_UCH * (NCH - 1), bhi, blo);
   mf_wait_vmem();
 
   // Border maps: with one staging batch they are written between the two
@@ -629,7 +646,7 @@ __global__ void __launch_bounds__(256, 2) k_dense_mfma(MfmaArgs a0, MfBatch bt) 
     if constexpr (BATCH) {
       if (rt != cur) {  // another request: its Toeplitz fragments
         mf_tload(a.ttab, 0, ahi, alo);
-        if constexpr (NC > 1) mf_tload(a.ttab, 1, bhi, blo);
+        if constexpr (NC > 1) mf_tload_rows(a.ttab, 1, NU - MF_UCH * (NCH - 1), bhi, blo);
         mf_wait_vmem();
         cur = rt;
       }
@@ -643,7 +660,7 @@ __global__ void __launch_bounds__(256, 2) k_dense_mfma(MfmaArgs a0, MfBatch bt) 
       if constexpr (NC == 1) {
         mf_chunk<G>(lw, ahi, alo, acc);
       } else if constexpr (NC == 2 && NCH == 2) {
-        mf_rows16<G>(lw, ahi, alo, bhi, blo, acc);
+        mf_rows16<G, NU>(lw, ahi, alo, bhi, blo, acc);
       } else if constexpr (NC == 2) {
         mf_chunk<G>(lw, ahi, alo, acc);
         mf_chunk<G>(lw + 32, bhi, blo, acc);
@@ -681,9 +698,9 @@ struct MfVariant {
   size_t lds;
 };
 
-template <int KB, int NCH>
+template <int KB, int NCH, int NU = MF_UCH * NCH>
 static MfVariant mf_variant() {
-  return {k_dense_mfma<KB, NCH, false>, k_dense_mfma<KB, NCH, true>,
+  return {k_dense_mfma<KB, NCH, false, NU>, k_dense_mfma<KB, NCH, true, NU>,
           sizeof(uint32_t) * (size_t)MfmaGeom<KB, NCH>::LDS_WORDS};
 }
 
@@ -854,6 +871,8 @@ hipError_t launch_dense_mfma(const Dense4 &in, Dense4 &out, const int64_t *out_f
   if (!mf_select(mfma_kb(k.n[0]), mfma_nch(k.n[1]), &v) ||
       !dense_mfma_args(in, out, out_first, k, d_table, d_taps, kinv, style, fill, &a))
     return hipErrorInvalidValue;
+  // a 15-row kernel in one K block sweeps 15 rows, not the padded 16
+  if (mfma_kb(k.n[0]) == 1 && k.n[1] == 15) v = mf_variant<1, 2, 15>();
   MfBatch none = {nullptr, nullptr, nullptr};
   void *args[] = {&a, &none};
   return hipLaunchKernel((const void *)v.fn, dim3((unsigned)mf_grid(a.ntiles)), dim3(256), args, v.lds, st);

@@ -1208,6 +1208,74 @@ static int window_edge_check(ifm_ctx *ctx, int style, const Dense4 &in, const De
   return IFM_OK;
 }
 
+int ifm_mapwindow_generic(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, int op, double param,
+                          const int64_t *idx_first, const int64_t *idx_step, const int64_t *win_first,
+                          const int64_t *win_len, const ifm_border *b, int *nan_seen) {
+  if (!ctx) return IFM_E_ARG;
+  Geo g;
+  int rc;
+  if ((rc = common_setup(ctx, in, out, g))) return rc;
+  if ((rc = check_border(ctx, b))) return rc;
+  if (op < IFM_WIN_SUM || op > IFM_WIN_QUANTILE) return fail(ctx, IFM_E_ARG, "unknown window function");
+  if (op == IFM_WIN_QUANTILE && (!nan_seen || !(param >= 0.0 && param <= 1.0)))
+    return fail(ctx, IFM_E_ARG, "quantile needs p in [0, 1] and a nan_seen flag");
+  if (!idx_first || !idx_step) return fail(ctx, IFM_E_ARG, "indices are NULL");
+  const int dt = in->dtype, odt = out->dtype;
+  const bool same = odt == dt;
+  bool ok;
+  switch (op) {
+  case IFM_WIN_MIN:
+  case IFM_WIN_MAX:
+  case IFM_WIN_FIRST: ok = same; break;
+  case IFM_WIN_MEDIAN_FAST: ok = same || ((dt == IFM_U8 || dt == IFM_N0F8) && (odt == IFM_F32 || odt == IFM_F64)); break;
+  case IFM_WIN_QUANTILE: ok = odt == IFM_F64 && (dt == IFM_F32 || dt == IFM_F64 || dt == IFM_U8); break;
+  case IFM_WIN_MEDIAN:
+    ok = ((dt == IFM_F32 || dt == IFM_F64 || dt == IFM_BF16 || dt == IFM_F16) && same) || (dt == IFM_U8 && odt == IFM_F64);
+    break;
+  default: ok = ((dt == IFM_F32 || dt == IFM_F64) && same) || (dt == IFM_U8 && odt == IFM_F64); break;
+  }
+  if (!ok) return fail(ctx, IFM_E_UNSUPPORTED, "window function / eltype pair not provided on the GPU");
+  int wf[MAXD], wl[MAXD];
+  if ((rc = window_setup(ctx, g, win_first, win_len, wf, wl))) return rc;
+  if ((int64_t)wl[0] * wl[1] * wl[2] * wl[3] > 4096) return fail(ctx, IFM_E_UNSUPPORTED, "window larger than 4096 elements");
+  View vi, vo;
+  if ((rc = make_view(ctx, in, g, S_IN, true, vi))) return rc;
+  if ((rc = make_view(ctx, out, g, S_OUT, false, vo))) return rc;
+  int64_t f[MAXD], st[MAXD];
+  to_canon(g, idx_first, 0, f);
+  to_canon(g, idx_step, 1, st);
+  for (int m = 0; m < MAXD; ++m) {
+    if (vo.d.n[m] == 0) continue;
+    const int64_t last = f[m] + (vo.d.n[m] - 1) * st[m];
+    if (st[m] == 0 || f[m] < 0 || f[m] >= vi.d.n[m] || last < 0 || last >= vi.d.n[m])
+      return fail(ctx, IFM_E_DIM, "window centres lie outside the image axes");
+    if (b->style == IFM_FILL || (wf[m] == 0 && wl[m] == 1)) continue;
+    for (int64_t i = 0; i < vo.d.n[m]; ++i) {  // edge windows need a usable index map
+      const int64_t q = f[m] + i * st[m], lo = q + wf[m], hi = q + wf[m] + wl[m] - 1;
+      if (lo >= 0 && hi <= vi.d.n[m] - 1) continue;
+      const int64_t len = std::min(hi, vi.d.n[m] - 1) - std::max(lo, (int64_t)0) + 1;
+      if (len <= 0 || (b->style == IFM_REFLECT && len == 1))
+        return fail(ctx, IFM_E_ARG, "window does not overlap the image enough for this border (empty or zero-period index map)");
+    }
+  }
+  if (!ctx->finite_flag && hipMalloc(&ctx->finite_flag, sizeof(int)) != hipSuccess) {
+    ctx->finite_flag = nullptr;
+    return fail(ctx, IFM_E_NOMEM, "device allocation failed (flag)");
+  }
+  HIPCHK(hipMemsetAsync(ctx->finite_flag, 0, sizeof(int), ctx->stream));
+  HIPCHK(launch_window_generic(vi.d, vo.d, f, st, wf, wl, b->style, b->fill, op, param, ctx->finite_flag,
+                               ctx->stream));
+  if ((rc = finish_view(ctx, vo))) return rc;
+  if (nan_seen) {
+    int h = 0;
+    HIPCHK(hipMemcpyAsync(&h, ctx->finite_flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    *nan_seen = h ? 1 : 0;
+    return IFM_OK;
+  }
+  return sync_if_host(ctx, !in->device || !out->device);
+}
+
 int ifm_mapwindow_median(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, const int64_t *out_first,
                          const int64_t *win_first, const int64_t *win_len, const ifm_border *b) {
   if (!ctx) return IFM_E_ARG;

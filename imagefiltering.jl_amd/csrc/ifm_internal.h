@@ -277,6 +277,9 @@ struct MedianArgs {
   int odt;  // output eltype: dt, or F32 / F64 for UInt8 / N0f8 codes
   uint32_t fillkey;
 };
+hipError_t launch_window_generic(const Dense4 &in, Dense4 &out, const int64_t *first, const int64_t *step,
+                                 const int *wfirst, const int *wlen, int style, double fill, int op, double param,
+                                 int *flag, hipStream_t st);
 hipError_t launch_median(const Dense4 &in, Dense4 &out, const int64_t *out_first, const int *wfirst,
                          const int *wlen, int style, double fill, hipStream_t st);
 // The circular edge band of a centred k x k median (after the network kernel).

@@ -15,6 +15,8 @@
 // retires batches as their events complete.  Its per-call mode runs the same
 // requests one at a time through the single-image entry points.
 #include <chrono>
+#include <cstring>
+#include <deque>
 #include <map>
 #include <thread>
 
@@ -736,6 +738,172 @@ int ifm_serve(ifm_ctx *ctx, const ifm_request *reqs, const double *arrival_s, in
   }
   (void)hipStreamSynchronize(ctx->stream);
   if (abase) (void)hipFree(abase);
+  return rc;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Request server: non-blocking submit / poll over the batch machinery above.
+// ---------------------------------------------------------------------------
+struct ifm_server {
+  ifm_ctx *ctx = nullptr;
+  uint64_t budget = 0;
+  int max_batch = 1024, max_inflight = 2;
+  struct Req {
+    ifm_request rq;
+    uint64_t ticket = 0, bytes = 0;
+    std::vector<double> taps;  // the request's taps, owned here
+  };
+  std::deque<Req> pending;
+  struct Flight {
+    hipEvent_t ev;
+    uint64_t last_ticket, bytes;
+  };
+  std::deque<Flight> flights;
+  uint64_t next_ticket = 1, done_below = 1, inflight_bytes = 0;
+};
+
+namespace {
+
+int srv_fail(ifm_server *s, int code, const std::string &msg) { return sfail(s->ctx, code, msg); }
+
+// Retires completed batches (they complete in launch order) and launches the
+// pending requests the limits allow.  Never waits for the device.
+int srv_pump(ifm_server *s) {
+  while (!s->flights.empty()) {
+    const hipError_t q = hipEventQuery(s->flights.front().ev);
+    if (q == hipErrorNotReady) break;
+    if (q != hipSuccess) return srv_fail(s, IFM_E_DEVICE, std::string("batch: ") + hipGetErrorString(q));
+    s->done_below = s->flights.front().last_ticket + 1;
+    s->inflight_bytes -= s->flights.front().bytes;
+    s->flights.pop_front();
+  }
+  while (!s->pending.empty() && (int)s->flights.size() < s->max_inflight) {
+    std::vector<ifm_request> rq;
+    uint64_t bytes = 0;
+    for (const auto &r : s->pending) {
+      if ((int)rq.size() >= s->max_batch) break;
+      if (s->budget && s->inflight_bytes + bytes + r.bytes > s->budget) break;
+      rq.push_back(r.rq);
+      bytes += r.bytes;
+    }
+    if (rq.empty()) break;  // waits for batches in flight to free budget
+    std::vector<int> idx(rq.size());
+    for (size_t i = 0; i < idx.size(); ++i) idx[i] = (int)i;
+    Block blk;
+    std::vector<Group> groups;
+    Extra ex;
+    int rc = build_batch(s->ctx, rq.data(), idx.data(), (int)rq.size(), nullptr, blk, groups, ex);
+    if (rc) return rc;
+    ifm_server::Flight f;
+    rc = launch_batch(s->ctx, rq.data(), idx.data(), blk, groups, ex, &f.ev);
+    if (rc) return rc;
+    f.last_ticket = s->pending[rq.size() - 1].ticket;
+    f.bytes = bytes;
+    s->inflight_bytes += bytes;
+    s->flights.push_back(f);
+    for (size_t i = 0; i < rq.size(); ++i) s->pending.pop_front();
+  }
+  return IFM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ifm_server_create(ifm_ctx *ctx, const ifm_server_config *cfg, ifm_server **server) {
+  if (!ctx || !server) return ctx ? sfail(ctx, IFM_E_ARG, "server is NULL") : IFM_E_ARG;
+  ifm_server *s = new ifm_server;
+  s->ctx = ctx;
+  if (cfg) {
+    s->budget = cfg->budget_bytes;
+    if (cfg->max_batch > 0) s->max_batch = cfg->max_batch;
+    if (cfg->max_inflight > 0) s->max_inflight = std::min(cfg->max_inflight, NSLOTS - 1);
+  }
+  *server = s;
+  return IFM_OK;
+}
+
+int ifm_server_submit(ifm_server *s, const ifm_request *req, uint64_t *ticket) {
+  if (!s || !req || !ticket) return s ? srv_fail(s, IFM_E_ARG, "request or ticket is NULL") : IFM_E_ARG;
+  if (!req->in.data || req->in.device != 1 || !req->out.data || req->out.device != 1)
+    return srv_fail(s, IFM_E_ARG, "server requests take device arrays");
+  ifm_server::Req r;
+  r.rq = *req;
+  // the taps move into the server (the caller's may not outlive the call)
+  const bool sep = req->op == IFM_OP_SEPARABLE, dense = req->op == IFM_OP_DENSE;
+  size_t nt = 0;
+  if (sep) {
+    if (req->nfactors < 1 || req->nfactors > 2 || req->nfactors2 < 0 || req->nfactors2 > 2)
+      return srv_fail(s, IFM_E_ARG, "separable requests take 1 or 2 factors");
+    for (int i = 0; i < req->nfactors; ++i) nt += (size_t)std::max(req->factors[i].len, 0);
+    for (int i = 0; i < req->nfactors2; ++i) nt += (size_t)std::max(req->factors2[i].len, 0);
+  } else if (dense) {
+    size_t k = 1;
+    for (int d = 0; d < req->kernel.ndim && d < IFM_MAXDIM; ++d) k *= (size_t)std::max<int64_t>(req->kernel.size[d], 0);
+    nt = k;
+  }
+  r.taps.resize(nt);
+  size_t off = 0;
+  auto take = [&](const double *src, int64_t n) -> const double * {
+    if (n <= 0) return nullptr;
+    if (!src) return nullptr;
+    std::memcpy(r.taps.data() + off, src, (size_t)n * sizeof(double));
+    const double *p = r.taps.data() + off;
+    off += (size_t)n;
+    return p;
+  };
+  if (sep) {
+    for (int i = 0; i < req->nfactors; ++i) r.rq.factors[i].taps = take(req->factors[i].taps, req->factors[i].len);
+    for (int i = 0; i < req->nfactors2; ++i) r.rq.factors2[i].taps = take(req->factors2[i].taps, req->factors2[i].len);
+  } else if (dense) {
+    r.rq.kernel.taps = take(req->kernel.taps, (int64_t)nt);
+  }
+  // checked now, as a batch of one (no launch), so errors reach the caller
+  {
+    int i0 = 0;
+    Block blk;
+    std::vector<Group> groups;
+    Extra ex;
+    const int rc = build_batch(s->ctx, &r.rq, &i0, 1, nullptr, blk, groups, ex);
+    if (rc) return rc;
+  }
+  r.bytes = array_bytes(req->in) + array_bytes(req->out) + ((sep && req->nfactors2 > 0) ? array_bytes(req->out2) : 0);
+  if (s->budget && r.bytes > s->budget) return srv_fail(s, IFM_E_BUDGET, "a request exceeds the server's budget");
+  r.ticket = s->next_ticket++;
+  *ticket = r.ticket;
+  s->pending.push_back(std::move(r));
+  return srv_pump(s);
+}
+
+int ifm_server_poll(ifm_server *s, uint64_t ticket, int *done) {
+  if (!s || !done) return s ? srv_fail(s, IFM_E_ARG, "done is NULL") : IFM_E_ARG;
+  if (ticket == 0 || ticket >= s->next_ticket) return srv_fail(s, IFM_E_ARG, "unknown ticket");
+  const int rc = srv_pump(s);
+  if (rc) return rc;
+  *done = ticket < s->done_below ? 1 : 0;
+  return IFM_OK;
+}
+
+int ifm_server_wait(ifm_server *s, uint64_t ticket) {
+  if (!s) return IFM_E_ARG;
+  if (ticket >= s->next_ticket) return srv_fail(s, IFM_E_ARG, "unknown ticket");
+  const uint64_t want = ticket ? ticket : s->next_ticket - 1;
+  ifm_ctx *ctx = s->ctx;  // for SHIP
+  for (;;) {
+    int rc = srv_pump(s);
+    if (rc) return rc;
+    if (want < s->done_below) return IFM_OK;
+    if (s->flights.empty()) return srv_fail(s, IFM_E_DEVICE, "nothing in flight for the ticket");
+    SHIP(hipEventSynchronize(s->flights.front().ev));
+  }
+}
+
+int ifm_server_destroy(ifm_server *s) {
+  if (!s) return IFM_OK;
+  const int rc = s->next_ticket > 1 ? ifm_server_wait(s, 0) : IFM_OK;
+  delete s;
   return rc;
 }
 

@@ -1071,3 +1071,209 @@ hipError_t launch_minmax(const Dense4 &in, Dense4 *omin, Dense4 *omax, const int
 }
 
 }  // namespace ifm
+
+namespace ifm {
+
+// ---------------------------------------------------------------------------
+// Generic window functions at (possibly strided) window centres:
+// mapwindow(f, img, window; border, indices) for f in {sum, mean, minimum,
+// maximum, first, median!, Statistics.median, Statistics.quantile(., p)}
+// (src/mapwindow.jl:75-121, :270-333; the centres are the affine index map
+// of _IndexTransformer, :160-189).  One thread per output; the window is
+// enumerated on the fly with the window-relative edge maps of copy_win!
+// (window_key), and order statistics use counting selection on isless keys.
+// ---------------------------------------------------------------------------
+struct WinGenArgs {
+  MedianAnyArgs m;          // image, window, border; m.of = first centre, m.o = output extents
+  int step[MAXD];           // centre step per axis
+  int op;
+  double param;
+  int *flag;                // set when quantile meets a NaN (the reference throws)
+};
+
+// value of an isless key of eltype dt (F64 keys are 64-bit)
+template <typename KT>
+__device__ __forceinline__ double key_value(KT k, int dt) {
+  if constexpr (sizeof(KT) == 8) {
+    const uint64_t u = (k == ~0ull) ? 0x7ff8000000000000ull : ((k >> 63) ? (k & 0x7fffffffffffffffull) : ~k);
+    return __longlong_as_double((long long)u);
+  } else {
+    switch (dt) {
+    case IFM_U8: return (double)k;
+    case IFM_N0F8: return (double)k / 255.0;
+    case IFM_F32: {
+      const uint32_t u = (k == 0xffffffffu) ? 0x7fc00000u : ((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+      return (double)__uint_as_float(u);
+    }
+    default: {
+      uint16_t h;
+      store_key32(&h, 0, dt, k);
+      return dt == IFM_BF16 ? (double)__uint_as_float((uint32_t)h << 16)
+                            : (double)(float)__builtin_bit_cast(_Float16, h);
+    }
+    }
+  }
+}
+
+template <typename KT>
+__device__ __forceinline__ bool key_isnan(KT k, int dt) {
+  if constexpr (sizeof(KT) == 8) return k == ~0ull;
+  else return dt == IFM_F32 ? k == 0xffffffffu : ((dt == IFM_BF16 || dt == IFM_F16) ? k == 0xffffu : false);
+}
+
+template <typename KT>
+__global__ void __launch_bounds__(256) k_window_generic(WinGenArgs g) {
+  const MedianAnyArgs &a = g.m;
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < a.total;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    int64_t r = p, oidx = a.obase;
+    int q[MAXD], ilo[MAXD], ilen[MAXD];
+#pragma unroll
+    for (int d = 0; d < MAXD; ++d) {
+      const int ql = (int)(r % a.o[d]);
+      oidx += ql * a.ostride[d];
+      q[d] = a.of[d] + ql * g.step[d];
+      r /= a.o[d];
+      const int lo = q[d] + a.wf[d], hi = q[d] + a.wf[d] + a.wl[d] - 1;
+      ilo[d] = lo < 0 ? 0 : lo;
+      const int h2 = hi > a.n[d] - 1 ? a.n[d] - 1 : hi;
+      ilen[d] = h2 - ilo[d] + 1;
+    }
+    const int L = a.L;
+    switch (g.op) {
+    case IFM_WIN_FIRST: {
+      const KT k = window_key<KT>(a, q, ilo, ilen, 0);
+      if constexpr (sizeof(KT) == 8) store_key64(a.out, oidx, k);
+      else store_key_as(a.out, oidx, a.dt, a.odt, k);
+      break;
+    }
+    case IFM_WIN_MIN:
+    case IFM_WIN_MAX: {
+      // min / max propagate NaN (Julia's min, max)
+      KT best = window_key<KT>(a, q, ilo, ilen, 0);
+      bool nan = key_isnan(best, a.dt);
+      for (int j = 1; j < L; ++j) {
+        const KT k = window_key<KT>(a, q, ilo, ilen, j);
+        nan |= key_isnan(k, a.dt);
+        best = g.op == IFM_WIN_MIN ? (k < best ? k : best) : (k > best ? k : best);
+      }
+      if (nan) best = (sizeof(KT) == 8 || a.dt == IFM_F32) ? (KT)~(KT)0 : (KT)0xffffu;  // the NaN key
+      if constexpr (sizeof(KT) == 8) store_key64(a.out, oidx, best);
+      else store_key_as(a.out, oidx, a.dt, a.odt, best);
+      break;
+    }
+    case IFM_WIN_SUM:
+    case IFM_WIN_MEAN: {
+      double s = 0.0;
+      for (int j = 0; j < L; ++j) s += key_value<KT>(window_key<KT>(a, q, ilo, ilen, j), a.dt);
+      store_d(a.out, oidx, a.odt, g.op == IFM_WIN_MEAN ? s / (double)L : s);
+      break;
+    }
+    default: {  // order statistics
+      bool nan = false;
+      if (g.op != IFM_WIN_MEDIAN_FAST)
+        for (int j = 0; j < L; ++j) nan |= key_isnan(window_key<KT>(a, q, ilo, ilen, j), a.dt);
+      if (nan && g.op == IFM_WIN_QUANTILE) {
+        atomicOr(g.flag, 1);
+        store_d(a.out, oidx, a.odt, __longlong_as_double(0x7ff8000000000000ll));
+        break;
+      }
+      if (nan) {  // Statistics.median of a window holding NaN is NaN
+        store_d(a.out, oidx, a.odt, __longlong_as_double(0x7ff8000000000000ll));
+        break;
+      }
+      // ranks r1 <= r2 (0-based) and the interpolation weight between them
+      int r1, r2;
+      double gam = 0.0;
+      if (g.op == IFM_WIN_MEDIAN_FAST) {
+        r1 = r2 = (L + 1) / 2 - 1;
+      } else if (g.op == IFM_WIN_MEDIAN) {
+        r1 = (L - 1) / 2;
+        r2 = L / 2;
+      } else {
+        const double h = (double)(L - 1) * g.param;
+        r1 = (int)floor(h);
+        if (r1 > L - 1) r1 = L - 1;
+        if (r1 < 0) r1 = 0;
+        r2 = r1 + 1 < L ? r1 + 1 : r1;
+        gam = h - (double)r1;
+      }
+      KT k1 = 0, k2 = 0;
+      bool f1 = false, f2 = false;
+      for (int i = 0; i < L && !(f1 && f2); ++i) {
+        const KT ki = window_key<KT>(a, q, ilo, ilen, i);
+        int less = 0, eq = 0;
+        for (int j = 0; j < L; ++j) {
+          const KT kj = window_key<KT>(a, q, ilo, ilen, j);
+          less += kj < ki;
+          eq += kj == ki;
+        }
+        if (!f1 && less <= r1 && r1 < less + eq) {
+          k1 = ki;
+          f1 = true;
+        }
+        if (!f2 && less <= r2 && r2 < less + eq) {
+          k2 = ki;
+          f2 = true;
+        }
+      }
+      if (g.op == IFM_WIN_MEDIAN_FAST || (g.op == IFM_WIN_MEDIAN && r1 == r2)) {
+        if constexpr (sizeof(KT) == 8) store_key64(a.out, oidx, k1);
+        else store_key_as(a.out, oidx, a.dt, a.odt, k1);
+      } else if (g.op == IFM_WIN_MEDIAN) {
+        // Statistics.middle(x, y) = x/2 + y/2, in Float32 for Float32 values
+        const double x = key_value<KT>(k1, a.dt), y = key_value<KT>(k2, a.dt);
+        if (a.odt == IFM_F32 || a.odt == IFM_BF16 || a.odt == IFM_F16)
+          store_d(a.out, oidx, a.odt, (double)((float)x / 2.0f + (float)y / 2.0f));
+        else store_d(a.out, oidx, a.odt, x / 2.0 + y / 2.0);
+      } else {
+        const double x = key_value<KT>(k1, a.dt), y = key_value<KT>(k2, a.dt);
+        store_d(a.out, oidx, a.odt, x + gam * (y - x));
+      }
+    }
+    }
+  }
+}
+
+hipError_t launch_window_generic(const Dense4 &in, Dense4 &out, const int64_t *first, const int64_t *step,
+                                 const int *wfirst, const int *wlen, int style, double fill, int op, double param,
+                                 int *flag, hipStream_t st) {
+  WinGenArgs g;
+  MedianAnyArgs &a = g.m;
+  a.in = in.data;
+  a.out = out.data;
+  a.L = 1;
+  a.total = 1;
+  a.obase = 0;
+  int64_t stride = 1;
+  for (int d = 0; d < MAXD; ++d) {
+    a.n[d] = (int)in.n[d];
+    a.o[d] = (int)out.n[d];
+    a.of[d] = (int)first[d];
+    g.step[d] = (int)step[d];
+    a.wf[d] = wfirst[d];
+    a.wl[d] = wlen[d];
+    a.L *= wlen[d];
+    a.total *= out.n[d];
+    a.ostride[d] = stride;
+    stride *= out.n[d];
+  }
+  if (a.total == 0) return hipSuccess;
+  a.style = style;
+  a.dt = in.dtype;
+  a.odt = out.dtype;
+  a.fill = fill;
+  g.op = op;
+  g.param = param;
+  g.flag = flag;
+  int64_t blocks = (a.total + 255) / 256;
+  if (blocks > 1 << 20) blocks = 1 << 20;
+  if (in.dtype == IFM_F64)
+    hipLaunchKernelGGL(k_window_generic<uint64_t>, dim3((unsigned)blocks), dim3(256), 0, st, g);
+  else
+    hipLaunchKernelGGL(k_window_generic<uint32_t>, dim3((unsigned)blocks), dim3(256), 0, st, g);
+  return hipGetLastError();
+}
+
+}  // namespace ifm
+

@@ -21,14 +21,16 @@ from .imfilter import (CPU1, FFT, FIR, IIR, AbstractResource, Algorithm, ColorIm
                        Mixed, channelview, colorview, factorkernel, filter_algorithm, filter_type, imfilter, imfilter_,
                        kernelconv)
 from .kernelfactors import kernelfactors
-from .mapwindow import extrema, mapwindow, mapwindow_, maximum, median, median_, minimum, resolve_window
+from .mapwindow import (extrema, first, mapwindow, mapwindow_, maximum, mean, median, median_, minimum, quantile,
+                        resolve_window, sum_)
 from .offsetarrays import OffsetArray, centered
 from .specialty import BlobLoG, blob_LoG, findlocalmaxima, findlocalminima, imgradients, multiLoG
 
 __all__ = [
     "Kernel", "KernelFactors", "Pad", "Fill", "Inner", "NA", "NoPad", "BorderArray", "Algorithm",
     "imfilter", "imfilter_", "mapwindow", "mapwindow_", "padarray", "centered", "kernelfactors",
-    "OffsetArray", "DeviceArray", "median", "median_", "minimum", "maximum", "extrema",
+    "OffsetArray", "DeviceArray", "median", "median_", "minimum", "maximum", "extrema", "mean", "sum_", "first",
+    "quantile",
     "imgradients", "findlocalmaxima", "findlocalminima", "blob_LoG", "multiLoG", "BlobLoG",
     "ColorImage", "colorview", "channelview",
 ]

@@ -62,6 +62,10 @@ class IfmRequest(C.Structure):
                 ("factors2", IfmFactor * 2)]
 
 
+class IfmServerConfig(C.Structure):
+    _fields_ = [("budget_bytes", C.c_uint64), ("max_batch", C.c_int32), ("max_inflight", C.c_int32)]
+
+
 class IfmServeConfig(C.Structure):
     _fields_ = [("budget_bytes", C.c_uint64), ("mode", C.c_int32), ("max_batch", C.c_int32),
                 ("max_inflight", C.c_int32), ("reserved", C.c_int32)]
@@ -79,8 +83,9 @@ EXPORTS = [
     "ifm_ctx_create", "ifm_ctx_destroy", "ifm_ctx_set_stream", "ifm_ctx_synchronize", "ifm_last_error",
     "ifm_version", "ifm_padarray", "ifm_imfilter_fir", "ifm_imfilter_fir_noalloc", "ifm_imfilter_dense", "ifm_all_finite", "ifm_na_prepare", "ifm_divide",
     "ifm_copy_into", "ifm_local_extrema", "ifm_convert_int", "ifm_imfilter_fft",
-    "ifm_imfilter_iir", "ifm_mapwindow_median", "ifm_mapwindow_minmax", "ifm_extrema_filter",
+    "ifm_imfilter_iir", "ifm_mapwindow_median", "ifm_mapwindow_minmax", "ifm_extrema_filter", "ifm_mapwindow_generic",
     "ifm_batch_run", "ifm_serve", "ifm_fill_uniform",
+    "ifm_server_create", "ifm_server_submit", "ifm_server_poll", "ifm_server_wait", "ifm_server_destroy",
     "ifm_device_alloc", "ifm_device_free", "ifm_memcpy_h2d", "ifm_memcpy_d2h", "ifm_memset_d",
     "ifm_timer_start", "ifm_timer_stop", "ifm_device_count", "ifm_set_tile_rules", "ifm_tile_height",
 ]
@@ -127,10 +132,17 @@ def load(path=LIBPATH):
     L.ifm_mapwindow_minmax.argtypes = [vp, P(IfmArray), P(IfmArray), P(IfmArray), P(C.c_int64), P(C.c_int64),
                                        P(C.c_int64), P(IfmBorder)]
     L.ifm_extrema_filter.argtypes = [vp, P(IfmArray), P(IfmArray), P(IfmArray), P(C.c_int64)]
+    L.ifm_mapwindow_generic.argtypes = [vp, P(IfmArray), P(IfmArray), C.c_int, C.c_double, P(C.c_int64),
+                                        P(C.c_int64), P(C.c_int64), P(C.c_int64), P(IfmBorder), P(C.c_int)]
     L.ifm_batch_run.argtypes = [vp, P(IfmRequest), C.c_int]
     L.ifm_serve.argtypes = [vp, P(IfmRequest), P(C.c_double), C.c_int, P(IfmServeConfig), P(C.c_double),
                             P(C.c_double)]
     L.ifm_fill_uniform.argtypes = [vp, P(IfmArray), C.c_uint64]
+    L.ifm_server_create.argtypes = [vp, P(IfmServerConfig), P(vp)]
+    L.ifm_server_submit.argtypes = [vp, P(IfmRequest), P(C.c_uint64)]
+    L.ifm_server_poll.argtypes = [vp, C.c_uint64, P(C.c_int)]
+    L.ifm_server_wait.argtypes = [vp, C.c_uint64]
+    L.ifm_server_destroy.argtypes = [vp]
     L.ifm_device_alloc.argtypes = [vp, C.c_uint64, P(vp)]
     L.ifm_device_free.argtypes = [vp, vp]
     L.ifm_memcpy_h2d.argtypes = [vp, vp, vp, C.c_uint64]

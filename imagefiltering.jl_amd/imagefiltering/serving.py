@@ -170,6 +170,48 @@ def serve(mix, mode="batched", budget_bytes=64 << 30, max_batch=1024, max_inflig
     return admit, done
 
 
+class Server:
+    """Non-blocking request server (ifm_server_*): ``submit`` queues one
+    request (an ``IfmRequest`` over device arrays) and returns its ticket
+    without waiting for the GPU; requests submitted while earlier batches run
+    join the next batch.  ``poll(ticket)`` reports completion without
+    waiting, ``wait(ticket)`` blocks (ticket 0: everything submitted)."""
+
+    def __init__(self, budget_bytes=0, max_batch=1024, max_inflight=2):
+        self._L, ctx = _lib.load(), _lib.ctx()
+        cfg = _lib.IfmServerConfig()
+        cfg.budget_bytes = int(budget_bytes)
+        cfg.max_batch = int(max_batch)
+        cfg.max_inflight = int(max_inflight)
+        self._h = C.c_void_p()
+        _lib.check(self._L.ifm_server_create(ctx, C.byref(cfg), C.byref(self._h)))
+
+    def submit(self, req):
+        t = C.c_uint64()
+        _lib.check(self._L.ifm_server_submit(self._h, C.byref(req), C.byref(t)))
+        return t.value
+
+    def poll(self, ticket):
+        d = C.c_int(0)
+        _lib.check(self._L.ifm_server_poll(self._h, int(ticket), C.byref(d)))
+        return bool(d.value)
+
+    def wait(self, ticket=0):
+        _lib.check(self._L.ifm_server_wait(self._h, int(ticket)))
+
+    def close(self):
+        if self._h:
+            rc = self._L.ifm_server_destroy(self._h)
+            self._h = C.c_void_p()
+            _lib.check(rc)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
 def stats(mix, done):
     """images/s, Gpixel/s, HBM GB/s and latency percentiles (arrival to done)."""
     t_end = float(np.max(done)) if len(done) else 0.0

@@ -222,6 +222,36 @@ int ifm_local_extrema(ifm_ctx *ctx, const ifm_array *in, ifm_array *flags, const
                       const int32_t *edges, int sense);
 
 /* ---- mapwindow ------------------------------------------------------------ */
+/* Window functions of ifm_mapwindow_generic. */
+typedef enum {
+  IFM_WIN_SUM = 0,          /* sum (FP64 accumulation, stored in out's eltype) */
+  IFM_WIN_MEAN = 1,         /* Statistics.mean */
+  IFM_WIN_MIN = 2,          /* minimum (NaN propagates) */
+  IFM_WIN_MAX = 3,          /* maximum (NaN propagates) */
+  IFM_WIN_FIRST = 4,        /* first (the window's first element, column-major) */
+  IFM_WIN_MEDIAN_FAST = 5,  /* median! -> median_fast!: the ((L+1)/2)-th smallest under isless */
+  IFM_WIN_MEDIAN = 6,       /* Statistics.median: middle of the two central values for even L,
+                               NaN when the window holds a NaN */
+  IFM_WIN_QUANTILE = 7      /* Statistics.quantile(v, param), linear interpolation between
+                               order statistics; a window holding NaN sets *nan_seen */
+} ifm_window_op;
+
+/* Replaces mapwindow(f, img, window; border, indices) / mapwindow! with a
+ * window function from ifm_window_op, src/mapwindow.jl:75-121, :270-333: out[i]
+ * = f(window at centre idx_first + i * idx_step), per Julia axis, 0-based;
+ * out's extents are the numbers of centres (the strided `indices` ranges;
+ * _IndexTransformer, src/mapwindow.jl:156-189).  Edge windows map relative to
+ * their intersection with the image (copy_win!, :300-325), Fill substitutes
+ * the value.  out's eltype: in's for SUM / MEAN / MIN / MAX / FIRST / MEDIAN
+ * of floats, Float64 for UInt8 SUM / MEAN / MEDIAN / QUANTILE, Float32 or
+ * Float64 as typeof(middle(x)) for MEDIAN_FAST of UInt8 / N0f8, Float64 for
+ * QUANTILE.  Windows up to 4096 elements; every centre must lie inside the
+ * image.  nan_seen (may be NULL unless op is QUANTILE) reports a NaN met by
+ * QUANTILE, where the reference throws ArgumentError. */
+int ifm_mapwindow_generic(ifm_ctx *ctx, const ifm_array *in, ifm_array *out, int op, double param,
+                          const int64_t *idx_first, const int64_t *idx_step, const int64_t *win_first,
+                          const int64_t *win_len, const ifm_border *border, int *nan_seen);
+
 /* Replaces mapwindow(median!, img, window; border) -> median_fast!
  * (src/mapwindow.jl:123-131, :270-333): the ((L+1)÷2)-th smallest of the
  * L window values under isless order.  win_first[d]/win_len[d] give the window
@@ -294,6 +324,39 @@ typedef struct {
  * since the call started) at which request i was launched / seen complete. */
 int ifm_serve(ifm_ctx *ctx, const ifm_request *reqs, const double *arrival_s, int n,
               const ifm_serve_config *cfg, double *admit_s, double *done_s);
+
+/* ---- request server: non-blocking submit / poll ------------------------------ */
+/* No counterpart in the reference (its callers loop over imfilter /
+ * mapwindow).  A server batches the requests submitted to it and launches
+ * them on its ctx's stream without blocking the caller: a submitted request
+ * joins the pending batch, which is launched as soon as fewer than
+ * max_inflight batches are on the device and the bytes of the requests in
+ * flight (in + out, plus out2) stay within budget_bytes; requests submitted
+ * while others run join the next batch (continuous batching).  Tickets
+ * complete in submission order.  Requests carry device arrays that the
+ * caller keeps alive until their ticket completes; their taps are copied at
+ * submit.  One thread per server; a server owns its ctx's stream while it has
+ * requests in flight.  in.data == NULL (generated inputs) is not accepted. */
+typedef struct ifm_server ifm_server;
+
+typedef struct {
+  uint64_t budget_bytes;        /* in + out bytes of the requests in flight (0: no limit) */
+  int32_t max_batch;            /* most requests in one launch (0: 1024) */
+  int32_t max_inflight;         /* most batches on the device at once (0: 2; at most 7) */
+} ifm_server_config;
+
+int ifm_server_create(ifm_ctx *ctx, const ifm_server_config *cfg, ifm_server **server);
+/* Queues one request (checked as ifm_batch_run checks it) and returns its
+ * ticket; launches what the limits allow.  Never waits for the device. */
+int ifm_server_submit(ifm_server *server, const ifm_request *req, uint64_t *ticket);
+/* Retires completed batches, launches what the limits allow, and sets *done
+ * to 1 when the ticket's result is in its out array.  Never waits. */
+int ifm_server_poll(ifm_server *server, uint64_t ticket, int *done);
+/* Launches every pending request that fits the limits, then waits until the
+ * ticket completes (0: until every submitted request completes). */
+int ifm_server_wait(ifm_server *server, uint64_t ticket);
+/* Waits for everything submitted, then frees the server. */
+int ifm_server_destroy(ifm_server *server);
 
 /* Fills a device array with seeded uniform [0,1) values: the top 24 bits of a
  * counter hash times 2^-24 (exact in FP32; BF16 / FP16 round to nearest even). */

@@ -321,3 +321,76 @@ def imfilter_mixed(img, stages, border):
         sl = tuple(slice(inds[d][0] - cov[d][0], inds[d][1] - cov[d][0] + 1) for d in range(N))
         A = A[sl]
     return np.asfortranarray(A)
+
+
+def _isless_key(x):
+    """Sort key of Julia's isless on floats: NaN last, -0.0 before +0.0."""
+    x = float(x)
+    if np.isnan(x):
+        return (2, 0.0)
+    return (0 if (x < 0 or (x == 0 and np.signbit(x))) else 1, x)
+
+
+def mapwindow_generic(img, fname, wfirst, wlen, border, first, step, shape, p=None):
+    """mapwindow(f, img, window; border, indices) restated in numpy for the
+    window functions sum, mean, minimum, maximum, first, median! (median_fast!),
+    median (Statistics) and quantile(p) (src/mapwindow.jl:75-121, :270-333):
+    f at the window centres first + i * step (0-based), edge windows mapped
+    relative to their intersection with the image (copy_win! with
+    padindex(border, inner, outer), :300-325), Fill substituting the value.
+    Results in FP64 (selected elements keep their values)."""
+    a = np.asarray(img, dtype=np.float64)
+    N = a.ndim
+    style, fill = (border, None) if isinstance(border, str) else ("fill", float(border[1]))
+    out = np.empty(tuple(shape), dtype=np.float64)
+    for oi in np.ndindex(*shape):
+        axes_idx, axes_fill = [], []
+        for d in range(N):
+            q = first[d] + oi[d] * step[d]
+            lo, hi = q + wfirst[d], q + wfirst[d] + wlen[d] - 1
+            pos = np.arange(lo, hi + 1)
+            if style == "fill":
+                ok = (pos >= 0) & (pos < a.shape[d])
+                axes_idx.append(np.where(ok, pos, 0))
+                axes_fill.append(~ok)
+            else:
+                a0, b0 = max(lo, 0), min(hi, a.shape[d] - 1)
+                idx = padindex(style, a0 - lo, b0 - a0 + 1, hi - b0) + a0
+                axes_idx.append(idx)
+                axes_fill.append(np.zeros(len(pos), bool))
+        w = a[np.ix_(*axes_idx)]
+        if style == "fill":
+            mask = np.zeros(w.shape, bool)
+            for d in range(N):
+                sh = [1] * N
+                sh[d] = -1
+                mask |= axes_fill[d].reshape(sh)
+            w = np.where(mask, fill, w)
+        v = w.ravel(order="F")  # column-major window order, as the Julia buffer
+        if fname == "sum":
+            r = v.sum()
+        elif fname == "mean":
+            r = v.mean()
+        elif fname == "minimum":
+            r = np.nan if np.isnan(v).any() else min(v, key=_isless_key)
+        elif fname == "maximum":
+            r = np.nan if np.isnan(v).any() else max(v, key=_isless_key)
+        elif fname == "first":
+            r = v[0]
+        elif fname == "median!":
+            r = sorted(v, key=_isless_key)[(len(v) + 1) // 2 - 1]
+        elif fname == "median":
+            if np.isnan(v).any():
+                r = np.nan
+            else:
+                s = np.sort(v)
+                n = len(s)
+                r = s[n // 2] if n % 2 else s[n // 2 - 1] / 2 + s[n // 2] / 2
+        elif fname == "quantile":
+            if np.isnan(v).any():
+                raise ValueError("quantiles are undefined in presence of NaNs")
+            r = np.quantile(v, p)
+        else:
+            raise ValueError(fname)
+        out[oi] = r
+    return out

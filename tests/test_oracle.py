@@ -461,3 +461,72 @@ def test_dense_kernel_with_fewer_dims_extends():
     k = rng.standard_normal(3)
     np.testing.assert_array_equal(O.imfilter_dense(img, k, [-1], "reflect"),
                                   O.imfilter_dense(img, k.reshape(3, 1), [-1, 0], "reflect"))
+
+
+# ---- generic mapwindow -----------------------------------------------------------------
+def test_mapwindow_generic_reference_median_vectors():
+    """test/mapwindow.jl:105-124 known answers for median and median!."""
+    for c in GOLD["median"]["cases"]:
+        x = np.asarray(c["input"], dtype=np.float64)
+        w = c["window"]
+        if x.ndim == 1:
+            wf, wl = [w[0]], [w[1] - w[0] + 1]
+        else:
+            wf, wl = [-(w[0] // 2), -(w[1] // 2)], list(w)
+        for f in ("median", "median!"):
+            got = O.mapwindow_generic(x, f, wf, wl, "replicate", [0] * x.ndim, [1] * x.ndim, list(x.shape))
+            np.testing.assert_array_equal(got, np.asarray(c["expect"], dtype=np.float64), err_msg=str((f, c)))
+
+
+def test_mapwindow_generic_matches_scipy():
+    """Unit-step windows against scipy.ndimage (uniform mean, sum via
+    correlate with ones, min / max / median filters) for the Pad modes whose
+    window-relative edge map equals the image-wide one on 3-wide windows
+    (circular wraps inside the window's intersection instead, which
+    orc_median_f64's own pins cover)."""
+    rng = np.random.default_rng(5)
+    a = rng.standard_normal((9, 7))
+    for style in ("replicate", "symmetric", "reflect"):
+        m = ND_MODE[style]
+        for fname, ref in [("mean", ndi.uniform_filter(a, 3, mode=m)),
+                           ("sum", ndi.correlate(a, np.ones((3, 3)), mode=m)),
+                           ("minimum", ndi.minimum_filter(a, 3, mode=m)),
+                           ("maximum", ndi.maximum_filter(a, 3, mode=m)),
+                           ("median!", ndi.median_filter(a, 3, mode=m))]:
+            got = O.mapwindow_generic(a, fname, [-1, -1], [3, 3], style, [0, 0], [1, 1], [9, 7])
+            np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-12, err_msg=str((style, fname)))
+
+
+def test_mapwindow_generic_strided_is_subsample_and_inner_first():
+    """test/mapwindow.jl:131-162: f at strided indices equals the full result
+    subsampled; first over (0:2,) returns the image at the centres."""
+    rng = np.random.default_rng(6)
+    a = rng.standard_normal((10, 5))
+    full = O.mapwindow_generic(a, "mean", [-1, 0], [3, 1], "replicate", [0, 0], [1, 1], [10, 5])
+    sub = O.mapwindow_generic(a, "mean", [-1, 0], [3, 1], "replicate", [0, 0], [2, 1], [4, 3])
+    np.testing.assert_array_equal(sub, full[0:8:2, 0:3])
+    img48 = 10.0 * np.arange(1, 11)
+    got = O.mapwindow_generic(img48, "first", [0], [3], "replicate", [1], [2], [4])
+    np.testing.assert_array_equal(got, img48[1:9:2])
+    got = O.mapwindow_generic(img48, "minimum", [-2], [3], "replicate", [2], [1], [8])
+    np.testing.assert_array_equal(got, img48[0:8])
+
+
+def test_mapwindow_generic_even_median_and_quantile():
+    """Statistics.median of an even window is the middle of the two central
+    values; quantile interpolates linearly (type 7, numpy's default); a
+    4-element window pins both by hand."""
+    v = np.array([4.0, 1.0, 3.0, 2.0])
+    assert O.mapwindow_generic(v, "median", [0], [4], "replicate", [0], [1], [1])[0] == 2.5
+    assert O.mapwindow_generic(v, "quantile", [0], [4], "replicate", [0], [1], [1], p=0.25)[0] == 1.75
+    assert O.mapwindow_generic(v, "median!", [0], [4], "replicate", [0], [1], [1])[0] == 2.0
+    nanv = np.array([1.0, np.nan, 3.0])
+    assert np.isnan(O.mapwindow_generic(nanv, "median", [-1], [3], "replicate", [1], [1], [1])[0])
+    assert O.mapwindow_generic(nanv, "median!", [-1], [3], "replicate", [1], [1], [1])[0] == 3.0
+    # circular edge windows wrap inside their intersection with the image,
+    # as the C oracle's median does
+    rng = np.random.default_rng(7)
+    c = rng.integers(0, 9, (7, 6)).astype(np.float64)
+    np.testing.assert_array_equal(
+        O.mapwindow_generic(c, "median!", [-1, -2], [3, 5], "circular", [0, 0], [1, 1], [7, 6]),
+        O.median(c, [-1, -2], [3, 5], "circular"))

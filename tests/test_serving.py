@@ -154,6 +154,67 @@ def test_batched_equals_alone_and_percall_and_oracle():
 
 
 @pytest.mark.gpu
+def test_server_submit_poll_alone_equals_mix():
+    """ifm_server_submit / poll: every request's result is the same submitted
+    alone (submit, wait) and submitted in a stream with the others (submits
+    interleaved with polls, requests joining batches while earlier ones run),
+    bit for bit, and equal to ifm_batch_run's; a small budget and batch
+    limit force several batches in flight."""
+    mix = _small_mix(24, seed=21)
+    bufs, keep = _buffers(mix)
+    arr, k2 = serving.requests(mix, bufs)
+    n = mix["n"]
+    alone = []
+    with serving.Server(budget_bytes=0, max_batch=8) as srv:
+        for i in range(n):
+            t = srv.submit(arr[i])
+            srv.wait(t)
+            assert srv.poll(t)
+            alone.append(_outputs(keep)[i])
+    for i in range(n):  # scramble the outputs so the stream has to rewrite them
+        _lib.check(_lib.load().ifm_memset_d(_lib.ctx(), keep[3 * i + 1].ptr, 0xFF, keep[3 * i + 1].nbytes))
+    need = int(serving.request_bytes(mix).max())
+    with serving.Server(budget_bytes=3 * need, max_batch=5, max_inflight=3) as srv:
+        tickets = []
+        for i in range(n):
+            tickets.append(srv.submit(arr[i]))
+            srv.poll(tickets[0])
+        assert tickets == sorted(tickets) and len(set(tickets)) == n
+        srv.wait(tickets[n // 2])
+        assert all(srv.poll(t) for t in tickets[:n // 2 + 1])
+        srv.wait()
+        assert all(srv.poll(t) for t in tickets)
+    mixed = _outputs(keep)
+    for i in range(n):
+        np.testing.assert_array_equal(np.asarray(mixed[i][0]), np.asarray(alone[i][0]), err_msg="request %d" % i)
+        if alone[i][1] is not None:
+            np.testing.assert_array_equal(mixed[i][1], alone[i][1])
+    L, ctx = _lib.load(), _lib.ctx()
+    _lib.check(L.ifm_batch_run(ctx, arr, n))
+    _lib.check(L.ifm_ctx_synchronize(ctx))
+    batch = _outputs(keep)
+    for i in range(n):
+        np.testing.assert_array_equal(np.asarray(batch[i][0]), np.asarray(mixed[i][0]), err_msg="request %d" % i)
+
+
+@pytest.mark.gpu
+def test_server_rejects_bad_requests_at_submit():
+    mix = _small_mix(4, seed=2)
+    bufs, keep = _buffers(mix)
+    arr, k2 = serving.requests(mix, bufs)
+    with serving.Server() as srv:
+        bad = (_lib.IfmRequest * 1)(arr[2])
+        bad[0].window = 9
+        with pytest.raises(NotImplementedError):
+            srv.submit(bad[0])
+        t = srv.submit(arr[0])
+        srv.wait(t)
+    with serving.Server(budget_bytes=16) as srv:
+        with pytest.raises(RuntimeError, match="budget"):
+            srv.submit(arr[0])
+
+
+@pytest.mark.gpu
 def test_serve_generated_inputs_under_a_small_budget():
     mix = serving.request_mix(60, rate=5000.0, seed=5, side=(64, 700))
     budget = 24 << 20

@@ -369,6 +369,20 @@ def run_extras(IF, _lib, L, ctx, T, img, d_img, d_out, b_reflect, args, cpu, hea
     rate("dense15_valu_fp32_4096", lambda: L.ifm_imfilter_dense(ctx, a_in, a_out, None, ks15, b_reflect, 1), n * n,
          8, 3)
     note("dense15_valu_fp32_4096", max_rel_err_vs_fp64=relerr(out_rows(d_out), want15))
+    # small non-separable kernels: the register-blocked VALU stencils
+    # (imfilter(Float32, img, centered(KxK), "reflect", Algorithm.FIR()))
+    for K in (5, 7):
+        kK = norm_kernel(np.random.default_rng(20261220 + K), K)
+        ksK = _kernel_struct(IF.centered(kK))
+        name = "dense%d_valu_4096" % K
+        rate(name, lambda ksK=ksK: L.ifm_imfilter_dense(ctx, a_in, a_out, None, ksK, b_reflect, 0), n * n, 8, 10,
+             kernel="k_dense2d_small<%d,%d>" % (K, K))
+        (wantK, tK, uK, pxK) = cpu.scaled(
+            lambda u, kK=kK, K=K: (lambda: O.imfilter_dense(img64, kK, [-(K // 2)] * 2, "reflect", [0, rows_r0],
+                                                            [n, u]), n * u), rows_n, 1, units0=8)
+        outK = np.asarray(d_out.to_host())[:, rows_r0:rows_r0 + uK]
+        note(name, max_rel_err_vs_fp64=relerr(outK, wantK),
+             cpu_baseline=cpu.entry(pxK, tK, 1, "%d x %d px, direct FP64 FIR" % (n, uK), "one thread, as CPU1(FIR())"))
 
     # the drop-in call through the public API with the default algorithm:
     # filter_algorithm's isfinite scan, then FFT

@@ -6,6 +6,7 @@
 //
 //  * k_dense_nd     any rank <= 4, one output per thread (small kernels, 3-D)
 //  * k_dense2d_valu 2-D, LDS tile, 8 rows per thread, FP32 FMA
+//  * k_dense2d_small the same, register-blocked for odd kernel sizes 3 .. 13
 // The matrix-core path for 2-D kernels is k_mfma.hip.
 #include "ifm_internal.h"
 
@@ -132,6 +133,100 @@ __global__ void __launch_bounds__(256) k_dense2d_valu(Dense2dArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// 2-D VALU, register-blocked for compile-time kernel sizes KX x KY (odd, 3 ..
+// 13): the same 64 x 32 tile and thread shape as k_dense2d_valu, but every
+// loop is unrolled, the taps arrive as kernel arguments (scalar registers,
+// v_fmac with an SGPR operand), each LDS value read feeds up to KY * 8 / (KY
+// + 7) FMAs through a sliding column of KX values, and an interior tile is
+// staged without border maps.
+// ---------------------------------------------------------------------------
+template <int KX, int KY>
+struct SmallTaps {
+  float k[KX * KY];  // column-major (axis 0 fastest)
+};
+
+template <int KX, int KY>
+__global__ void __launch_bounds__(256) k_dense2d_small(Dense2dArgs a, SmallTaps<KX, KY> kt) {
+  constexpr int IN_W = DV_TX + KX - 1, IN_H = DV_TY + KY - 1, IN_WP = IN_W | 1;
+  __shared__ float smem[IN_H * IN_WP];
+  const int tid = threadIdx.x;
+  const int tx = tid & 63, ty = tid >> 6;
+  const int x0 = a.f0 + blockIdx.x * DV_TX;
+  const int y0 = a.f1 + blockIdx.y * DV_TY;
+  const int gx = x0 + a.first0, gy = y0 + a.first1;  // image index of staged (0, 0)
+  const bool interior = gx >= 0 && gx + IN_W <= a.n0 && gy >= 0 && gy + IN_H <= a.n1;
+  for (int plane = blockIdx.z; plane < a.nplanes; plane += gridDim.z) {
+    const int p2 = plane % a.o2, p3 = plane / a.o2;
+    const int64_t ibase = ((int64_t)(p2 + a.f2) + (int64_t)a.n2 * (p3 + a.f3)) * a.in_plane;
+    if (interior) {
+      for (int r = ty; r < IN_H; r += 4) {
+        const int64_t rb = ibase + (int64_t)(gy + r) * a.n0 + gx;
+        for (int c = tx; c < IN_W; c += 64) smem[r * IN_WP + c] = load_f(a.in, rb + c, a.dt_in);
+      }
+    } else {
+      for (int r = ty; r < IN_H; r += 4) {
+        const int sy = bmap(a.style, gy + r, a.n1);
+        for (int c = tx; c < IN_W; c += 64) {
+          const int sx = bmap(a.style, gx + c, a.n0);
+          smem[r * IN_WP + c] = (sy < 0 || sx < 0) ? a.fill : load_f(a.in, ibase + (int64_t)sy * a.n0 + sx, a.dt_in);
+        }
+      }
+    }
+    __syncthreads();
+    float acc[DV_R];
+#pragma unroll
+    for (int o = 0; o < DV_R; ++o) acc[o] = 0.f;
+    const float *base = smem + ty * DV_R * IN_WP + tx;
+#pragma unroll
+    for (int u = 0; u < KY + DV_R - 1; ++u) {
+      float v[KX];
+#pragma unroll
+      for (int t = 0; t < KX; ++t) v[t] = base[u * IN_WP + t];
+#pragma unroll
+      for (int o = 0; o < DV_R; ++o) {
+        const int j = u - o;
+        if (j >= 0 && j < KY) {
+#pragma unroll
+          for (int t = 0; t < KX; ++t) acc[o] = fmaf(kt.k[t + j * KX], v[t], acc[o]);
+        }
+      }
+    }
+    const int ox = blockIdx.x * DV_TX + tx;
+    if (ox < a.o0) {
+      const int64_t obase = (int64_t)plane * a.out_plane;
+#pragma unroll
+      for (int o = 0; o < DV_R; ++o) {
+        const int oy = blockIdx.y * DV_TY + ty * DV_R + o;
+        if (oy < a.o1) store_f(a.out, obase + (int64_t)oy * a.o0 + ox, a.dt_out, acc[o]);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <int KX, int KY>
+static hipError_t launch_small(const Dense2dArgs &a, const float *taps, dim3 grid, hipStream_t st) {
+  SmallTaps<KX, KY> kt;
+  for (int i = 0; i < KX * KY; ++i) kt.k[i] = taps[i];
+  hipLaunchKernelGGL((k_dense2d_small<KX, KY>), grid, dim3(256), 0, st, a, kt);
+  return hipGetLastError();
+}
+
+template <int KX>
+static hipError_t launch_small_ky(int ky, const Dense2dArgs &a, const float *taps, dim3 grid, hipStream_t st) {
+  switch (ky) {
+  case 3: return launch_small<KX, 3>(a, taps, grid, st);
+  case 5: return launch_small<KX, 5>(a, taps, grid, st);
+  case 7: return launch_small<KX, 7>(a, taps, grid, st);
+  case 9: return launch_small<KX, 9>(a, taps, grid, st);
+  case 11: return launch_small<KX, 11>(a, taps, grid, st);
+  default: return launch_small<KX, 13>(a, taps, grid, st);
+  }
+}
+
+static bool small_size(int n) { return n >= 3 && n <= 13 && (n & 1); }
+
 hipError_t launch_dense_valu(const Dense4 &in, Dense4 &out, const int64_t *out_first, const DenseKernel &k,
                              const float *d_taps, int style, float fill, hipStream_t st) {
   if (out.count() == 0) return hipSuccess;
@@ -165,6 +260,18 @@ hipError_t launch_dense_valu(const Dense4 &in, Dense4 &out, const int64_t *out_f
     const int IN_W = DV_TX + a.k0 - 1, IN_H = DV_TY + a.k1 - 1;
     size_t lds = sizeof(float) * (size_t)IN_H * (IN_W | 1);
     dim3 grid((a.o0 + DV_TX - 1) / DV_TX, (a.o1 + DV_TY - 1) / DV_TY, a.nplanes < 65535 ? a.nplanes : 65535);
+    if (small_size(a.k0) && small_size(a.k1)) {
+      // register-blocked form; taps from the host copy of the kernel
+      const float *taps = k.taps.data();
+      switch (a.k0) {
+      case 3: return launch_small_ky<3>(a.k1, a, taps, grid, st);
+      case 5: return launch_small_ky<5>(a.k1, a, taps, grid, st);
+      case 7: return launch_small_ky<7>(a.k1, a, taps, grid, st);
+      case 9: return launch_small_ky<9>(a.k1, a, taps, grid, st);
+      case 11: return launch_small_ky<11>(a.k1, a, taps, grid, st);
+      default: return launch_small_ky<13>(a.k1, a, taps, grid, st);
+      }
+    }
     hipLaunchKernelGGL(k_dense2d_valu, grid, dim3(256), lds, st, a);
     return hipGetLastError();
   }

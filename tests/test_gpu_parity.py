@@ -249,6 +249,34 @@ def test_dense_valu_small_kernels(ks):
         assert relerr(out, want) <= TOL, (ks, b)
 
 
+@pytest.mark.parametrize("ks", [(3, 3), (5, 5), (7, 7), (9, 13), (13, 11), (3, 13), (11, 5), (13, 13)])
+def test_dense_register_blocked_sizes(ks):
+    """The register-blocked VALU kernels (odd sizes 3 .. 13 per axis):
+    kernels off centre, images with interior and border tiles, a 3-D image
+    (planes), F64 / BF16 / UInt8 inputs, every border."""
+    from imagefiltering import _lib
+    rng = np.random.default_rng(300 + 17 * ks[0] + ks[1])
+    k = rng.standard_normal(ks)
+    first = [-(ks[0] // 3), -(ks[1] - 2)]
+    img = np.asfortranarray(rng.random((150, 70)).astype(np.float32))
+    for b in STYLES + [IF.Fill(0.5)]:
+        out = _dense_lib(img, k, first, b if isinstance(b, IF.Fill) else IF.Pad(b), _lib.PREC_FP32)
+        want = O.imfilter_dense(img.astype(np.float64), k, first, obord(b))
+        assert relerr(out, want) <= TOL, (ks, b)
+    vol = np.asfortranarray(rng.random((70, 40, 3)))
+    kern = IF.OffsetArray(k, *first)
+    out = IF.imfilter(vol, kern, "reflect", IF.Algorithm.FIR())
+    want = O.imfilter_dense(vol, k, first, "reflect")
+    assert relerr(out, want) <= TOL
+    codes = np.asfortranarray(rng.integers(0, 256, (67, 45)).astype(np.uint8))
+    out = IF.imfilter(codes, kern, "symmetric", IF.Algorithm.FIR())
+    assert relerr(out, O.imfilter_dense(codes.astype(np.float64), k, first, "symmetric")) <= TOL
+    xb = IF.bf16(rng.random((66, 70)).astype(np.float32))
+    out = IF.imfilter(IF.Float32, xb, kern, IF.Fill(0.0), IF.Algorithm.FIR())
+    want = O.imfilter_dense(IF.bf16_to_float32(xb).astype(np.float64), k, first, ("fill", 0.0))
+    assert relerr(out, want) <= TOL
+
+
 def norm_random_kernel(rng, K):
     while True:
         k = rng.standard_normal((K, K))

@@ -721,6 +721,9 @@ __global__ void __launch_bounds__(256) k_minmax_pass(MinmaxPassArgs a, KT fillke
 // is op(m[x], m[x + w - p]) for the largest power of two p <= w.
 // ---------------------------------------------------------------------------
 constexpr int MM_TX = 64, MM_TY = 32, MM_MAXW = 33;
+// Row pitch of the axis-0 results: odd, so the 8 rows one wave writes at once
+// fall on different banks.
+constexpr int MM_TXP = MM_TX + 1;
 
 struct Minmax2dArgs {
   const void *in;
@@ -770,7 +773,7 @@ __global__ void __launch_bounds__(256) k_minmax2d(Minmax2dArgs a) {
   const int iw = MM_TX + w0 - 1, ih = MM_TY + w1 - 1, iwp = iw | 1;
   uint32_t *s_in = mm_lds;                 // ih x iwp raw keys
   uint32_t *s_n = mm_lds + ih * iwp;       // ih x 64 min keys after axis 0
-  uint32_t *s_x = s_n + ih * MM_TX;        // ih x 64 max keys after axis 0
+  uint32_t *s_x = s_n + ih * MM_TXP;       // ih x 64 max keys after axis 0 (pitch 65)
   const int tid = threadIdx.x;
   const int ox0 = blockIdx.x * MM_TX, oy0 = blockIdx.y * MM_TY;
   const int gx0 = a.f0 + ox0 + a.wf0, gy0 = a.f1 + oy0 + a.wf1;
@@ -860,8 +863,8 @@ __global__ void __launch_bounds__(256) k_minmax2d(Minmax2dArgs a) {
       }
 #pragma unroll
       for (int o = 0; o < 8; ++o) {
-        s_n[r * MM_TX + c0 + o] = on[o];
-        s_x[r * MM_TX + c0 + o] = ox[o];
+        s_n[r * MM_TXP + c0 + o] = on[o];
+        s_x[r * MM_TXP + c0 + o] = ox[o];
       }
     }
     __syncthreads();
@@ -873,8 +876,8 @@ __global__ void __launch_bounds__(256) k_minmax2d(Minmax2dArgs a) {
         uint32_t vn[8 + W1 - 1], vx[8 + W1 - 1];
 #pragma unroll
         for (int t = 0; t < 8 + W1 - 1; ++t) {
-          vn[t] = s_n[(r0 + t) * MM_TX + col];
-          vx[t] = s_x[(r0 + t) * MM_TX + col];
+          vn[t] = s_n[(r0 + t) * MM_TXP + col];
+          vx[t] = s_x[(r0 + t) * MM_TXP + col];
         }
         win_minmax<W1, 8>(vn, vx, on, ox);
       } else {
@@ -884,7 +887,7 @@ __global__ void __launch_bounds__(256) k_minmax2d(Minmax2dArgs a) {
           ox[o] = 0u;
         }
         for (int t = 0; t < w1 + 7; ++t) {
-          const uint32_t kn = s_n[(r0 + t) * MM_TX + col], kx = s_x[(r0 + t) * MM_TX + col];
+          const uint32_t kn = s_n[(r0 + t) * MM_TXP + col], kx = s_x[(r0 + t) * MM_TXP + col];
 #pragma unroll
           for (int o = 0; o < 8; ++o)
             if (t - o >= 0 && t - o < w1) {
@@ -965,7 +968,7 @@ static bool minmax2d(const Dense4 &in, Dense4 *omin, Dense4 *omax, const int64_t
   a.out_plane = ref.n[0] * ref.n[1];
   a.nplanes = (int)(ref.n[2] * ref.n[3]);
   const int iw = MM_TX + a.wl0 - 1, ih = MM_TY + a.wl1 - 1;
-  const size_t lds = sizeof(uint32_t) * ((size_t)ih * (iw | 1) + 2 * (size_t)ih * MM_TX);
+  const size_t lds = sizeof(uint32_t) * ((size_t)ih * (iw | 1) + 2 * (size_t)ih * MM_TXP);
   dim3 grid((a.o0 + MM_TX - 1) / MM_TX, (a.o1 + MM_TY - 1) / MM_TY, a.nplanes < 65535 ? a.nplanes : 65535);
   switch (in.dtype) {
   case IFM_F32: mm_dispatch<IFM_F32>(a, grid, lds, st); break;

@@ -137,9 +137,16 @@ __global__ void __launch_bounds__(256) k_dense2d_valu(Dense2dArgs a) {
 // 2-D VALU, register-blocked for compile-time kernel sizes KX x KY (odd, 3 ..
 // 13): the same 64 x 32 tile and thread shape as k_dense2d_valu, but every
 // loop is unrolled, the taps arrive as kernel arguments (scalar registers,
-// v_fmac with an SGPR operand), each LDS value read feeds up to KY * 8 / (KY
-// + 7) FMAs through a sliding column of KX values, and an interior tile is
-// staged without border maps.
+// v_fmac with an SGPR operand), and each LDS value read feeds up to KY * 8 /
+// (KY + 7) FMAs through a sliding column of KX values.
+//
+// Staging.  Element e = tid + 256 i of the (64 + KX - 1) x (32 + KY - 1) input
+// tile (row-major) is loaded by thread tid: a wave reads one or two
+// contiguous runs, and for FP32 input every load of the tile is issued before
+// the first LDS write.  A tile whose window reaches past the image edge maps
+// its rows and columns once per workgroup into LDS (one bmap per row and per
+// column, not per element), so border tiles cost about what interior tiles
+// do.
 // ---------------------------------------------------------------------------
 template <int KX, int KY>
 struct SmallTaps {
@@ -149,28 +156,55 @@ struct SmallTaps {
 template <int KX, int KY>
 __global__ void __launch_bounds__(256) k_dense2d_small(Dense2dArgs a, SmallTaps<KX, KY> kt) {
   constexpr int IN_W = DV_TX + KX - 1, IN_H = DV_TY + KY - 1, IN_WP = IN_W | 1;
+  constexpr int NE = IN_H * IN_W, NI = (NE + 255) / 256;
   __shared__ float smem[IN_H * IN_WP];
+  __shared__ int smap[IN_H + IN_W];  // border tiles: source row of tile row r, then source column of tile column c
   const int tid = threadIdx.x;
   const int tx = tid & 63, ty = tid >> 6;
-  const int x0 = a.f0 + blockIdx.x * DV_TX;
-  const int y0 = a.f1 + blockIdx.y * DV_TY;
-  const int gx = x0 + a.first0, gy = y0 + a.first1;  // image index of staged (0, 0)
+  const int gx = a.f0 + blockIdx.x * DV_TX + a.first0;  // image index of staged (0, 0)
+  const int gy = a.f1 + blockIdx.y * DV_TY + a.first1;
   const bool interior = gx >= 0 && gx + IN_W <= a.n0 && gy >= 0 && gy + IN_H <= a.n1;
+  if (!interior) {
+    for (int i = tid; i < IN_H + IN_W; i += 256)
+      smap[i] = i < IN_H ? bmap(a.style, gy + i, a.n1) : bmap(a.style, gx + i - IN_H, a.n0);
+    __syncthreads();
+  }
   for (int plane = blockIdx.z; plane < a.nplanes; plane += gridDim.z) {
     const int p2 = plane % a.o2, p3 = plane / a.o2;
     const int64_t ibase = ((int64_t)(p2 + a.f2) + (int64_t)a.n2 * (p3 + a.f3)) * a.in_plane;
-    if (interior) {
-      for (int r = ty; r < IN_H; r += 4) {
-        const int64_t rb = ibase + (int64_t)(gy + r) * a.n0 + gx;
-        for (int c = tx; c < IN_W; c += 64) smem[r * IN_WP + c] = load_f(a.in, rb + c, a.dt_in);
+    if (a.dt_in == IFM_F32) {
+      const float *src = (const float *)a.in + ibase;
+      float x[NI];
+      if (interior) {
+        const float *t0 = src + (int64_t)gy * a.n0 + gx;
+#pragma unroll
+        for (int i = 0; i < NI; ++i) {
+          const int e = tid + 256 * i;
+          const int r = e / IN_W, c = e - r * IN_W;
+          if (e < NE) x[i] = t0[(int64_t)r * a.n0 + c];
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < NI; ++i) {
+          const int e = tid + 256 * i;
+          const int r = e / IN_W, c = e - r * IN_W;
+          if (e < NE) {
+            const int sy = smap[r], sx = smap[IN_H + c];
+            x[i] = (sy < 0 || sx < 0) ? a.fill : src[(int64_t)sy * a.n0 + sx];
+          }
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < NI; ++i) {
+        const int e = tid + 256 * i;
+        const int r = e / IN_W, c = e - r * IN_W;
+        if (e < NE) smem[r * IN_WP + c] = x[i];
       }
     } else {
-      for (int r = ty; r < IN_H; r += 4) {
-        const int sy = bmap(a.style, gy + r, a.n1);
-        for (int c = tx; c < IN_W; c += 64) {
-          const int sx = bmap(a.style, gx + c, a.n0);
-          smem[r * IN_WP + c] = (sy < 0 || sx < 0) ? a.fill : load_f(a.in, ibase + (int64_t)sy * a.n0 + sx, a.dt_in);
-        }
+      for (int e = tid; e < NE; e += 256) {
+        const int r = e / IN_W, c = e - r * IN_W;
+        const int sy = interior ? gy + r : smap[r], sx = interior ? gx + c : smap[IN_H + c];
+        smem[r * IN_WP + c] = (sy < 0 || sx < 0) ? a.fill : load_f(a.in, ibase + (int64_t)sy * a.n0 + sx, a.dt_in);
       }
     }
     __syncthreads();
@@ -182,23 +216,32 @@ __global__ void __launch_bounds__(256) k_dense2d_small(Dense2dArgs a, SmallTaps<
     for (int u = 0; u < KY + DV_R - 1; ++u) {
       float v[KX];
 #pragma unroll
-      for (int t = 0; t < KX; ++t) v[t] = base[u * IN_WP + t];
+      for (int q = 0; q < KX; ++q) v[q] = base[u * IN_WP + q];
 #pragma unroll
       for (int o = 0; o < DV_R; ++o) {
         const int j = u - o;
         if (j >= 0 && j < KY) {
 #pragma unroll
-          for (int t = 0; t < KX; ++t) acc[o] = fmaf(kt.k[t + j * KX], v[t], acc[o]);
+          for (int q = 0; q < KX; ++q) acc[o] = fmaf(kt.k[q + j * KX], v[q], acc[o]);
         }
       }
     }
     const int ox = blockIdx.x * DV_TX + tx;
     if (ox < a.o0) {
       const int64_t obase = (int64_t)plane * a.out_plane;
+      if (a.dt_out == IFM_F32) {
+        float *out = (float *)a.out + obase + ox;
 #pragma unroll
-      for (int o = 0; o < DV_R; ++o) {
-        const int oy = blockIdx.y * DV_TY + ty * DV_R + o;
-        if (oy < a.o1) store_f(a.out, obase + (int64_t)oy * a.o0 + ox, a.dt_out, acc[o]);
+        for (int o = 0; o < DV_R; ++o) {
+          const int oy = blockIdx.y * DV_TY + ty * DV_R + o;
+          if (oy < a.o1) out[(int64_t)oy * a.o0] = acc[o];
+        }
+      } else {
+#pragma unroll
+        for (int o = 0; o < DV_R; ++o) {
+          const int oy = blockIdx.y * DV_TY + ty * DV_R + o;
+          if (oy < a.o1) store_f(a.out, obase + (int64_t)oy * a.o0 + ox, a.dt_out, acc[o]);
+        }
       }
     }
     __syncthreads();

@@ -922,6 +922,181 @@ __global__ void __launch_bounds__(256) k_minmax2d(Minmax2dArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Min / max of 8-bit keys (UInt8 and N0f8 codes) over a W x W window, W = 3,
+// 5, 7, 9: the separable doubling scheme of k_minmax2d on keys packed two to a
+// register, so each v_pk_min_u16 / v_pk_max_u16 serves two output rows.
+// Tile 64 x 64 outputs; packed row p holds input rows p and p + 32 (the two
+// halves of the tile), so both passes run element-wise on the halves.
+// Staging reads four codes per lane (aligned words) on tiles whose columns
+// lie inside the image, and results leave through LDS as 16-byte stores.
+// Lanes of the axis-0 pass run over packed rows (odd row pitch), so their LDS
+// accesses fall on distinct banks.  WANT: 1 minimum, 2 maximum, 3 both (one
+// after the other through the same LDS buffers).
+// ---------------------------------------------------------------------------
+constexpr int M8_TX = 64, M8_TY = 64, M8_HY = 32, M8_NT = 256;
+
+template <int W, int NO>
+__device__ __forceinline__ void win_min2(u16x2 (&v)[NO + W - 1], u16x2 (&o)[NO]) {
+  constexpr int L = NO + W - 1;
+  constexpr int P = W >= 8 ? 8 : W >= 4 ? 4 : W >= 2 ? 2 : 1;
+#pragma unroll
+  for (int s = 1; 2 * s <= P; s *= 2) {
+#pragma unroll
+    for (int x = 0; x + s < L; ++x) v[x] = kmin(v[x], v[x + s]);
+  }
+#pragma unroll
+  for (int i = 0; i < NO; ++i) o[i] = kmin(v[i], v[i + W - P]);
+}
+
+template <int W, int NO>
+__device__ __forceinline__ void win_max2(u16x2 (&v)[NO + W - 1], u16x2 (&o)[NO]) {
+  constexpr int L = NO + W - 1;
+  constexpr int P = W >= 8 ? 8 : W >= 4 ? 4 : W >= 2 ? 2 : 1;
+#pragma unroll
+  for (int s = 1; 2 * s <= P; s *= 2) {
+#pragma unroll
+    for (int x = 0; x + s < L; ++x) v[x] = kmax(v[x], v[x + s]);
+  }
+#pragma unroll
+  for (int i = 0; i < NO; ++i) o[i] = kmax(v[i], v[i + W - P]);
+}
+
+__device__ __forceinline__ u16x2 pack2(uint32_t lo, uint32_t hi) {
+  u16x2 r;
+  r.x = (uint16_t)lo;
+  r.y = (uint16_t)hi;
+  return r;
+}
+
+template <int W, int WANT>
+__global__ void __launch_bounds__(M8_NT) k_minmax2d_u8(Minmax2dArgs a) {
+  constexpr int IW = M8_TX + W - 1;        // input columns of a tile
+  constexpr int NW = (IW + 6) / 4;         // aligned words covering them (any shift 0..3)
+  constexpr int SIN = 4 * NW + 1;          // staged row pitch (packed keys), odd
+  constexpr int NP = M8_HY + W - 1;        // packed rows
+  constexpr int SHP = M8_TX + 1;           // pitch after the axis-0 pass
+  constexpr int NI = (NP * NW + M8_NT - 1) / M8_NT;
+  __shared__ u16x2 s_in[NP * SIN];
+  __shared__ u16x2 s_h[NP * SHP];
+  __shared__ uint32_t s_out[M8_TY * M8_TX / 4];
+  const int tid = threadIdx.x;
+  const int ox0 = blockIdx.x * M8_TX;
+  const int gx0 = a.f0 + ox0 + a.wf0;
+  const uint8_t *in = (const uint8_t *)a.in;
+  // words: columns inside the image, rows of 4-byte multiples, aligned base
+  const bool words = gx0 >= 0 && gx0 + IW <= a.n0 && (a.n0 & 3) == 0 && ((uintptr_t)a.in & 3) == 0;
+  const int xa = words ? (gx0 & ~3) : gx0;  // image column of staged column 0
+  const int sh = gx0 - xa;                  // staged column of tile column 0
+  const int nwr = (sh + IW + 3) >> 2;       // words a row needs (ends at or before the row's end)
+  const int oy0 = blockIdx.y * M8_TY;
+  const int gy0 = a.f1 + oy0 + a.wf1;
+  for (int plane = blockIdx.z; plane < a.nplanes; plane += gridDim.z) {
+    const int64_t ibase = (int64_t)plane * a.in_plane, obase = (int64_t)plane * a.out_plane;
+    {
+      if (words) {
+        // every load of the tile issued before the first LDS write
+        uint32_t w0[NI], w1[NI];
+#pragma unroll
+        for (int i = 0; i < NI; ++i) {
+          const int e = tid + M8_NT * i, p = e / NW, j = e - p * NW;
+          if (e < NP * NW && j < nwr) {
+            const int sy0 = min(max(gy0 + p, 0), a.n1 - 1), sy1 = min(max(gy0 + p + M8_HY, 0), a.n1 - 1);
+            w0[i] = *(const uint32_t *)(in + ibase + (int64_t)sy0 * a.n0 + xa + 4 * j);
+            w1[i] = *(const uint32_t *)(in + ibase + (int64_t)sy1 * a.n0 + xa + 4 * j);
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < NI; ++i) {
+          const int e = tid + M8_NT * i, p = e / NW, j = e - p * NW;
+          if (e < NP * NW && j < nwr) {
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+              s_in[p * SIN + 4 * j + b] = pack2((w0[i] >> (8 * b)) & 0xff, (w1[i] >> (8 * b)) & 0xff);
+          }
+        }
+      } else {
+        for (int e = tid; e < NP * IW; e += M8_NT) {
+          const int p = e / IW, c = e - p * IW;
+          const int sx = min(max(gx0 + c, 0), a.n0 - 1);
+          const int sy0 = min(max(gy0 + p, 0), a.n1 - 1), sy1 = min(max(gy0 + p + M8_HY, 0), a.n1 - 1);
+          s_in[p * SIN + c] = pack2(in[ibase + (int64_t)sy0 * a.n0 + sx], in[ibase + (int64_t)sy1 * a.n0 + sx]);
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        if (!(WANT & (1 << q))) continue;
+        const bool isMin = q == 0;
+        // axis 0: item = (packed row, group of 8 outputs), packed rows across lanes
+        for (int item = tid; item < NP * (M8_TX / 8); item += M8_NT) {
+          const int p = item % NP, c0 = (item / NP) * 8;
+          const u16x2 *src = s_in + p * SIN + sh + c0;
+          u16x2 v[8 + W - 1], o[8];
+#pragma unroll
+          for (int k = 0; k < 8 + W - 1; ++k) v[k] = src[k];
+          if (isMin) win_min2<W, 8>(v, o);
+          else win_max2<W, 8>(v, o);
+#pragma unroll
+          for (int i = 0; i < 8; ++i) s_h[p * SHP + c0 + i] = o[i];
+        }
+        __syncthreads();
+        // axis 1: thread = (column, group of 8 packed rows); bytes to s_out
+        {
+          const int col = tid % M8_TX, r0 = (tid / M8_TX) * 8;
+          const int gx = a.f0 + ox0 + col + a.wf0;
+          const bool xout = gx < 0 || gx + W - 1 > a.n0 - 1;
+          u16x2 v[8 + W - 1], o[8];
+#pragma unroll
+          for (int k = 0; k < 8 + W - 1; ++k) v[k] = s_h[(r0 + k) * SHP + col];
+          if (isMin) win_min2<W, 8>(v, o);
+          else win_max2<W, 8>(v, o);
+          uint8_t *so = (uint8_t *)s_out;
+#pragma unroll
+          for (int i = 0; i < 8; ++i) {
+            uint32_t lo = o[i].x, hi = o[i].y;
+            if (a.isfill) {
+              const int gyl = a.f1 + oy0 + r0 + i + a.wf1, gyh = gyl + M8_HY;
+              const uint32_t fk = a.fillkey;
+              if (xout || gyl < 0 || gyl + W - 1 > a.n1 - 1) lo = isMin ? min(lo, fk) : max(lo, fk);
+              if (xout || gyh < 0 || gyh + W - 1 > a.n1 - 1) hi = isMin ? min(hi, fk) : max(hi, fk);
+            }
+            so[(r0 + i) * M8_TX + col] = (uint8_t)lo;
+            so[(r0 + i + M8_HY) * M8_TX + col] = (uint8_t)hi;
+          }
+        }
+        __syncthreads();
+        // stores: thread = (row, 16 columns)
+        {
+          const int row = tid / (M8_TX / 16), c16 = (tid % (M8_TX / 16)) * 16;
+          const int oy = oy0 + row;
+          uint8_t *dst = (uint8_t *)(isMin ? a.omin : a.omax);
+          const bool vec = ox0 + M8_TX <= a.o0 && (a.o0 & 15) == 0 && ((uintptr_t)dst & 15) == 0;
+          if (oy < a.o1) {
+            const uint32_t *srow = s_out + (row * M8_TX + c16) / 4;
+            const int64_t off = obase + (int64_t)oy * a.o0 + ox0 + c16;
+            if (vec) {
+              *(uint4 *)(dst + off) = make_uint4(srow[0], srow[1], srow[2], srow[3]);
+            } else {
+              const uint8_t *sb = (const uint8_t *)srow;
+              for (int b = 0; b < 16; ++b)
+                if (ox0 + c16 + b < a.o0) dst[off + b] = sb[b];
+            }
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+template <int W>
+static void mm8_launch(const Minmax2dArgs &a, dim3 grid, hipStream_t st) {
+  if (a.omin && a.omax) hipLaunchKernelGGL((k_minmax2d_u8<W, 3>), grid, dim3(M8_NT), 0, st, a);
+  else if (a.omin) hipLaunchKernelGGL((k_minmax2d_u8<W, 1>), grid, dim3(M8_NT), 0, st, a);
+  else hipLaunchKernelGGL((k_minmax2d_u8<W, 2>), grid, dim3(M8_NT), 0, st, a);
+}
+
 template <int DT, int W>
 static void mm_launch(const Minmax2dArgs &a, dim3 grid, size_t lds, hipStream_t st) {
   hipLaunchKernelGGL((k_minmax2d<DT, W, W>), grid, dim3(256), lds, st, a);
@@ -970,6 +1145,18 @@ static bool minmax2d(const Dense4 &in, Dense4 *omin, Dense4 *omax, const int64_t
   const int iw = MM_TX + a.wl0 - 1, ih = MM_TY + a.wl1 - 1;
   const size_t lds = sizeof(uint32_t) * ((size_t)ih * (iw | 1) + 2 * (size_t)ih * MM_TXP);
   dim3 grid((a.o0 + MM_TX - 1) / MM_TX, (a.o1 + MM_TY - 1) / MM_TY, a.nplanes < 65535 ? a.nplanes : 65535);
+  const bool u8 = in.dtype == IFM_U8 || in.dtype == IFM_N0F8;
+  if (u8 && a.wl0 == a.wl1 && (a.wl0 == 3 || a.wl0 == 5 || a.wl0 == 7 || a.wl0 == 9)) {
+    const dim3 g8((a.o0 + M8_TX - 1) / M8_TX, (a.o1 + M8_TY - 1) / M8_TY, grid.z);
+    switch (a.wl0) {
+    case 3: mm8_launch<3>(a, g8, st); break;
+    case 5: mm8_launch<5>(a, g8, st); break;
+    case 7: mm8_launch<7>(a, g8, st); break;
+    default: mm8_launch<9>(a, g8, st); break;
+    }
+    *err = hipGetLastError();
+    return true;
+  }
   switch (in.dtype) {
   case IFM_F32: mm_dispatch<IFM_F32>(a, grid, lds, st); break;
   case IFM_BF16: mm_dispatch<IFM_BF16>(a, grid, lds, st); break;

@@ -853,6 +853,35 @@ def test_minmax_tiled_many_tiles(dtype):
                 assert np.signbit(gmn[200, 100, 2]) == np.signbit(mn[200, 100, 2])
 
 
+@pytest.mark.parametrize("w", [3, 5, 7, 9])
+def test_minmax_u8_packed_tiles(w):
+    """The packed 8-bit min/max tiles (UInt8 / N0f8 codes, square windows 3 ..
+    9): rows of 4-byte multiples (word staging) and not, output rows of
+    16-byte multiples (vector stores) and not, several planes, off-centre
+    windows, Fill, minimum and maximum alone, and extrema (both at once)."""
+    rng = np.random.default_rng(400 + w)
+    for shape in [(300, 170, 2), (1024, 40), (600, 70), (135, 66), (4, 9)]:
+        codes = rng.integers(0, 256, shape).astype(np.uint8)
+        codes[rng.random(shape) < 0.03] = 0
+        extra = [0] * (len(shape) - 2)
+        for wf in ([-(w // 2), -(w // 2)], [-(w - 1), 0], [0, -(w - 1)]):
+            window = tuple(range(f, f + w) for f in wf) + tuple(range(0, 1) for _ in extra)
+            centred = wf[0] == -(w // 2) and wf[1] == -(w // 2)
+            borders = ["replicate", IF.Fill(17)] + (["reflect"] if centred and min(shape[:2]) > w else [])
+            for b in borders:
+                mn, mx = O.minmax(codes.astype(np.float64), wf + extra, [w, w] + [1] * len(extra), obord(b))
+                for f, want in ((IF.minimum, mn), (IF.maximum, mx)):
+                    got = np.asarray(IF.mapwindow(f, codes, window, border=b))
+                    np.testing.assert_array_equal(got, want.astype(np.uint8), err_msg=str((shape, wf, b, f)))
+                if b == "replicate":
+                    got = np.asarray(IF.mapwindow(IF.maximum, IF.n0f8(codes), window, border=b)).view(np.uint8)
+                    np.testing.assert_array_equal(got, mx.astype(np.uint8), err_msg=str((shape, wf, "n0f8")))
+        mn, mx = O.extrema_lemire(codes, [w, w] + [1] * len(extra))
+        got = IF.mapwindow(IF.extrema, codes, (w, w) + (1,) * len(extra))
+        np.testing.assert_array_equal(np.asarray(got[..., 0]), mn.astype(np.uint8), str(shape))
+        np.testing.assert_array_equal(np.asarray(got[..., 1]), mx.astype(np.uint8), str(shape))
+
+
 def test_extrema_lemire_semantics():
     import json, os
     g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_fixtures.json")))

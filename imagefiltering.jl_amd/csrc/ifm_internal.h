@@ -40,6 +40,15 @@ __host__ __device__ inline int bmap(int style, int i, int n) {
   }
 }
 
+// Tile of workgroup `lin` (0 <= lin < n, the dispatch order) such that each
+// of the 8 XCDs, which receive workgroups round-robin (lin % 8), walks one
+// contiguous run of tiles: tiles that share cache lines (row neighbours,
+// halos) then meet in the same L2 instead of being fetched once per XCD.
+__device__ __forceinline__ int xcd_tile(int lin, int n) {
+  const int x = lin & 7, k = lin >> 3, per = n >> 3, rem = n & 7;
+  return x * per + min(x, rem) + k;
+}
+
 // N0f8 code / 255 rounded to nearest (Float32(x::N0f8)): the product with the
 // rounded reciprocal corrected by one FMA step, which equals the correctly
 // rounded quotient for every code 0..255 (checked exhaustively on the CPU).

@@ -146,7 +146,8 @@ __global__ void __launch_bounds__(256) k_dense2d_valu(Dense2dArgs a) {
 // the first LDS write.  A tile whose window reaches past the image edge maps
 // its rows and columns once per workgroup into LDS (one bmap per row and per
 // column, not per element), so border tiles cost about what interior tiles
-// do.
+// do.  Tiles are assigned XCD by XCD (xcd_tile), so the halo columns two
+// neighbouring tiles share are fetched into one L2.
 // ---------------------------------------------------------------------------
 template <int KX, int KY>
 struct SmallTaps {
@@ -161,8 +162,10 @@ __global__ void __launch_bounds__(256) k_dense2d_small(Dense2dArgs a, SmallTaps<
   __shared__ int smap[IN_H + IN_W];  // border tiles: source row of tile row r, then source column of tile column c
   const int tid = threadIdx.x;
   const int tx = tid & 63, ty = tid >> 6;
-  const int gx = a.f0 + blockIdx.x * DV_TX + a.first0;  // image index of staged (0, 0)
-  const int gy = a.f1 + blockIdx.y * DV_TY + a.first1;
+  const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  const int bx = tile % gridDim.x, by = tile / gridDim.x;
+  const int gx = a.f0 + bx * DV_TX + a.first0;  // image index of staged (0, 0)
+  const int gy = a.f1 + by * DV_TY + a.first1;
   const bool interior = gx >= 0 && gx + IN_W <= a.n0 && gy >= 0 && gy + IN_H <= a.n1;
   if (!interior) {
     for (int i = tid; i < IN_H + IN_W; i += 256)
@@ -226,20 +229,20 @@ __global__ void __launch_bounds__(256) k_dense2d_small(Dense2dArgs a, SmallTaps<
         }
       }
     }
-    const int ox = blockIdx.x * DV_TX + tx;
+    const int ox = bx * DV_TX + tx;
     if (ox < a.o0) {
       const int64_t obase = (int64_t)plane * a.out_plane;
       if (a.dt_out == IFM_F32) {
         float *out = (float *)a.out + obase + ox;
 #pragma unroll
         for (int o = 0; o < DV_R; ++o) {
-          const int oy = blockIdx.y * DV_TY + ty * DV_R + o;
+          const int oy = by * DV_TY + ty * DV_R + o;
           if (oy < a.o1) out[(int64_t)oy * a.o0] = acc[o];
         }
       } else {
 #pragma unroll
         for (int o = 0; o < DV_R; ++o) {
-          const int oy = blockIdx.y * DV_TY + ty * DV_R + o;
+          const int oy = by * DV_TY + ty * DV_R + o;
           if (oy < a.o1) store_f(a.out, obase + (int64_t)oy * a.o0 + ox, a.dt_out, acc[o]);
         }
       }

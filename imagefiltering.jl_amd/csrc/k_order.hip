@@ -9,6 +9,8 @@
 //  * k_median_any   any window / border (incl. circular, whose edge windows
 //                   wrap inside the window's intersection with the image,
 //                   src/mapwindow.jl:310-316): counting selection.
+//  * k_minmax2d     min / max / extrema in one pass over 2-D tiles; for 8-bit
+//                   codes and square windows 3..9, k_minmax2d_u8 (packed keys).
 //  * k_minmax_pass  one axis of the separable min/max over truncated windows.
 #include "ifm_internal.h"
 
@@ -931,7 +933,8 @@ __global__ void __launch_bounds__(256) k_minmax2d(Minmax2dArgs a) {
 // Staging reads four codes per lane (aligned words) on tiles whose columns
 // lie inside the image, and results leave through LDS as 16-byte stores.
 // Lanes of the axis-0 pass run over packed rows (odd row pitch), so their LDS
-// accesses fall on distinct banks.  WANT: 1 minimum, 2 maximum, 3 both (one
+// accesses fall on distinct banks.  Tiles are assigned XCD by XCD (xcd_tile):
+// a 64-byte tile row shares its cache lines with its neighbours.  WANT: 1 minimum, 2 maximum, 3 both (one
 // after the other through the same LDS buffers).
 // ---------------------------------------------------------------------------
 constexpr int M8_TX = 64, M8_TY = 64, M8_HY = 32, M8_NT = 256;
@@ -981,7 +984,9 @@ __global__ void __launch_bounds__(M8_NT) k_minmax2d_u8(Minmax2dArgs a) {
   __shared__ u16x2 s_h[NP * SHP];
   __shared__ uint32_t s_out[M8_TY * M8_TX / 4];
   const int tid = threadIdx.x;
-  const int ox0 = blockIdx.x * M8_TX;
+  const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  const int tx_ = tile % gridDim.x, ty_ = tile / gridDim.x;
+  const int ox0 = tx_ * M8_TX;
   const int gx0 = a.f0 + ox0 + a.wf0;
   const uint8_t *in = (const uint8_t *)a.in;
   // words: columns inside the image, rows of 4-byte multiples, aligned base
@@ -989,7 +994,7 @@ __global__ void __launch_bounds__(M8_NT) k_minmax2d_u8(Minmax2dArgs a) {
   const int xa = words ? (gx0 & ~3) : gx0;  // image column of staged column 0
   const int sh = gx0 - xa;                  // staged column of tile column 0
   const int nwr = (sh + IW + 3) >> 2;       // words a row needs (ends at or before the row's end)
-  const int oy0 = blockIdx.y * M8_TY;
+  const int oy0 = ty_ * M8_TY;
   const int gy0 = a.f1 + oy0 + a.wf1;
   for (int plane = blockIdx.z; plane < a.nplanes; plane += gridDim.z) {
     const int64_t ibase = (int64_t)plane * a.in_plane, obase = (int64_t)plane * a.out_plane;

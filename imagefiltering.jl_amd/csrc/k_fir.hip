@@ -264,8 +264,10 @@ __global__ void __launch_bounds__(256) k_sep2d(Sep2dArgs a) {
   t.dt_out = a.dt_out;
   t.fill = a.fill;
   t.ty = a.ty;
-  t.ox0 = blockIdx.x * S2_TX;
-  t.oy0 = blockIdx.y * a.ty;
+  // tiles XCD by XCD: row neighbours share their halo columns' cache lines
+  const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  t.ox0 = (tile % gridDim.x) * S2_TX;
+  t.oy0 = (tile / gridDim.x) * a.ty;
   t.x0 = a.f0 + t.ox0;
   t.y0 = a.f1 + t.oy0;
   for (int plane = blockIdx.z; plane < a.nplanes; plane += gridDim.z) {
@@ -316,7 +318,12 @@ inline size_t sep3d_lds_floats(int k0, int k1) { return sep2d_lds_floats(k0, k1)
 template <int K2, int DT>
 __global__ void __launch_bounds__(256) k_sep3d(Sep3dArgs a) {
   extern __shared__ float smem[];
-  const int zc = blockIdx.z % a.nzc, p3 = blockIdx.z / a.nzc;
+  // tiles XCD by XCD (x fastest, then y, then z chunk): neighbouring tiles
+  // share halo cache lines, which one L2 then serves
+  const int nxy = gridDim.x * gridDim.y;
+  const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y + nxy * blockIdx.z, nxy * gridDim.z);
+  const int bz = tile / nxy, bx = tile % gridDim.x, by = (tile % nxy) / gridDim.x;
+  const int zc = bz % a.nzc, p3 = bz / a.nzc;
   const int oz0 = zc * S3_NZ, nz = min(S3_NZ, a.o2 - oz0);
   Sep2dTile t;
   t.in = a.in;
@@ -334,8 +341,8 @@ __global__ void __launch_bounds__(256) k_sep3d(Sep3dArgs a) {
   t.dt_out = a.dt_out;
   t.fill = a.fill;
   t.ty = S2_TY;
-  t.ox0 = blockIdx.x * S2_TX;
-  t.oy0 = blockIdx.y * S2_TY;
+  t.ox0 = bx * S2_TX;
+  t.oy0 = by * S2_TY;
   t.x0 = a.f0 + t.ox0;
   t.y0 = a.f1 + t.oy0;
   const int in_w = S2_TX + a.k0 - 1, in_h = S2_TY + a.k1 - 1, in_wp = in_w | 1;

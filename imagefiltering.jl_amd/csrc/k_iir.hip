@@ -4,10 +4,11 @@
 // (:1027-1048) and rightborder! (:1051-1092): forward recursion with the
 // steady-state left initialisation, right-edge initialisation by the 3x3
 // matrix M (Triggs & Sdika Eqs. 14-15), backward recursion and scaling.
-// Lines along the contiguous axis run k_iir_rows (LDS-staged chunks); lines
-// along other axes run k_iir_seg (whole lines in registers, the axis split
-// across waves) up to 512 elements, k_iir_strided (one thread per line)
-// beyond.  Lanes sit on adjacent lines, which are adjacent in memory for the
+// Lines along the contiguous axis run k_iir_seg (whole lines in registers,
+// the axis split across waves, an LDS transpose) up to 512 elements and
+// k_iir_rows (LDS-staged chunks) beyond; lines along other axes run
+// k_iir_seg2 (two segments per wave, two workgroups per CU) up to 512
+// elements, k_iir_strided (one thread per line) beyond.  Lanes sit on adjacent lines, which are adjacent in memory for the
 // strided axes (coalesced).
 #include "ifm_internal.h"
 
@@ -463,6 +464,149 @@ __global__ void __launch_bounds__(64 * IS_MAXW) k_iir_seg(IirArgs p, IirSegTable
   }
 }
 
+// Strided lines, two segments per wave: the same segmented recursion as
+// k_iir_seg, but a workgroup owns 32 adjacent lines and lane l of wave w runs
+// segment 2w + (l >> 5) of line l & 31.  A workgroup then needs half the
+// waves for the same axis, so two workgroups share a CU (each wave still
+// holds a whole segment in registers) and one's loads and stores overlap the
+// other's recursions.  Each half-wave reads one contiguous run of 32 lines.
+constexpr int IS2_L = 32;  // lines per workgroup
+__host__ __device__ inline size_t iir_seg2_lds_doubles(int nw) { return 6 * IS_S + 2 * (size_t)nw * 3 * IS2_L; }
+
+template <int DT>
+__global__ void __launch_bounds__(32 * IS_MAXW) k_iir_seg2(IirArgs p, IirSegTables tb) {
+  extern __shared__ double sm[];
+  double *hs = sm, *gs = sm + 3 * IS_S;
+  double *stf = sm + 6 * IS_S, *stb = stf + tb.nw * 3 * IS2_L;
+  const int lane = threadIdx.x & 63;
+  const int ln = lane & (IS2_L - 1);
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nw = tb.nw, S = tb.seg;
+  const int sg = 2 * w + (lane >> 5);  // segment of this lane (>= nw: none)
+  const int n = p.n[p.axis];
+  const int s0 = sg * S;
+  const bool has = sg < nw;
+  const int L = has ? min(S, n - s0) : 0;
+  const bool first = sg == 0, last = sg == nw - 1;
+  const int64_t line = (int64_t)blockIdx.x * IS2_L + ln;
+  const bool ok = line < p.lines && has;
+  const int64_t st = p.stride;
+  const int64_t base = (line % st) + (line / st) * st * n + (int64_t)s0 * st;
+  double v[IS_S];
+#pragma unroll
+  for (int j = 0; j < IS_S; ++j)
+    if (j < L && ok) v[j] = iir_ld<DT>(p, base + (int64_t)j * st);
+  for (int i = threadIdx.x; i < 6 * IS_S; i += blockDim.x) sm[i] = i < 3 * IS_S ? tb.h[i / IS_S][i % IS_S]
+                                                                                   : tb.g[i / IS_S - 3][i % IS_S];
+  const double a1 = p.a[0], a2 = p.a[1], a3 = p.a[2];
+  const double b1 = p.b[0], b2 = p.b[1], b3 = p.b[2];
+  const double asum = a1 + a2 + a3, bsum = b1 + b2 + b3;
+  // forward: the first segment from y[-k] = u- (leftborder!), the others from zero
+  double ym1 = 0.0, ym2 = 0.0, ym3 = 0.0, xlast = 0.0;
+  if (first) {
+    const double iminus = p.isfill ? p.fill : v[0];
+    ym1 = ym2 = ym3 = iminus / (1 - asum);
+  }
+#pragma unroll
+  for (int j = 0; j < IS_S; ++j) {
+    if (j < L) {
+      if (j == L - 1) xlast = v[j];
+      const double y = v[j] + a1 * ym1 + a2 * ym2 + a3 * ym3;
+      v[j] = y;
+      ym3 = ym2;
+      ym2 = ym1;
+      ym1 = y;
+    }
+  }
+  if (has) {
+    stf[(sg * 3 + 0) * IS2_L + ln] = ym1;
+    stf[(sg * 3 + 1) * IS2_L + ln] = ym2;
+    stf[(sg * 3 + 2) * IS2_L + ln] = ym3;
+  }
+  __syncthreads();
+  // state entering segment sg: T_0 = end of segment 0, T_u = Phi T_{u-1} + P_u
+  // with Phi[r][c] = h_c[S - 1 - r]
+  double e0 = ym1, e1 = ym2, e2 = ym3;  // true end state of this segment
+  if (has && !first) {
+    double t0 = stf[ln], t1 = stf[IS2_L + ln], t2 = stf[2 * IS2_L + ln];
+    for (int u = 1; u < sg; ++u) {
+      const double *q = stf + u * 3 * IS2_L;
+      const double n0 = hs[S - 1] * t0 + hs[IS_S + S - 1] * t1 + hs[2 * IS_S + S - 1] * t2 + q[ln];
+      const double n1 = hs[S - 2] * t0 + hs[IS_S + S - 2] * t1 + hs[2 * IS_S + S - 2] * t2 + q[IS2_L + ln];
+      const double n2 = hs[S - 3] * t0 + hs[IS_S + S - 3] * t1 + hs[2 * IS_S + S - 3] * t2 + q[2 * IS2_L + ln];
+      t0 = n0;
+      t1 = n1;
+      t2 = n2;
+    }
+#pragma unroll
+    for (int j = 0; j < IS_S; ++j)
+      if (j < L) v[j] += hs[j] * t0 + hs[IS_S + j] * t1 + hs[2 * IS_S + j] * t2;
+    e0 = ym1 + hs[L - 1] * t0 + hs[IS_S + L - 1] * t1 + hs[2 * IS_S + L - 1] * t2;
+    e1 = ym2 + hs[L - 2] * t0 + hs[IS_S + L - 2] * t1 + hs[2 * IS_S + L - 2] * t2;
+    e2 = ym3 + hs[L - 3] * t0 + hs[IS_S + L - 3] * t1 + hs[2 * IS_S + L - 3] * t2;
+  }
+  // backward: the last segment from the rightborder! values (e = y[n-1],
+  // y[n-2], y[n-3]), the others from zero
+  double z0 = 0.0, z1 = 0.0, z2 = 0.0;
+  if (last) {
+    const double iplus = p.isfill ? p.fill : xlast;
+    const double uplus = iplus / (1 - asum);
+    const double vplus = uplus / (1 - bsum);
+    const double d0 = e0 - uplus, d1 = e1 - uplus, d2 = e2 - uplus;
+    const double m0 = p.M[0] * d0 + p.M[1] * d1 + p.M[2] * d2 + vplus;
+    const double m1 = p.M[3] * d0 + p.M[4] * d1 + p.M[5] * d2 + vplus;
+    const double m2 = p.M[6] * d0 + p.M[7] * d1 + p.M[8] * d2 + vplus;
+    z2 = m0;
+    z1 = e1 + b1 * z2 + b2 * m1 + b3 * m2;
+    z0 = e2 + b1 * z1 + b2 * z2 + b3 * m1;
+  }
+  double zp1 = 0.0, zp2 = 0.0, zp3 = 0.0;
+#pragma unroll
+  for (int j = IS_S - 1; j >= 0; --j) {
+    if (j < L) {
+      double z;
+      if (last && j == L - 1) z = z2;
+      else if (last && j == L - 2) z = z1;
+      else if (last && j == L - 3) z = z0;
+      else z = v[j] + b1 * zp1 + b2 * zp2 + b3 * zp3;
+      v[j] = z;
+      zp3 = zp2;
+      zp2 = zp1;
+      zp1 = z;
+    }
+  }
+  if (has) {
+    stb[(sg * 3 + 0) * IS2_L + ln] = zp1;
+    stb[(sg * 3 + 1) * IS2_L + ln] = zp2;
+    stb[(sg * 3 + 2) * IS2_L + ln] = zp3;
+  }
+  __syncthreads();
+  // state entering segment sg from the right: R_{nw-1} = left end of the last
+  // segment, R_u = Psi R_{u+1} + Q_u with Psi[r][c] = g_c[S - 1 - r]
+  if (has && !last) {
+    const double *q0 = stb + (nw - 1) * 3 * IS2_L;
+    double t0 = q0[ln], t1 = q0[IS2_L + ln], t2 = q0[2 * IS2_L + ln];
+    for (int u = nw - 2; u > sg; --u) {
+      const double *q = stb + u * 3 * IS2_L;
+      const double n0 = gs[S - 1] * t0 + gs[IS_S + S - 1] * t1 + gs[2 * IS_S + S - 1] * t2 + q[ln];
+      const double n1 = gs[S - 2] * t0 + gs[IS_S + S - 2] * t1 + gs[2 * IS_S + S - 2] * t2 + q[IS2_L + ln];
+      const double n2 = gs[S - 3] * t0 + gs[IS_S + S - 3] * t1 + gs[2 * IS_S + S - 3] * t2 + q[2 * IS2_L + ln];
+      t0 = n0;
+      t1 = n1;
+      t2 = n2;
+    }
+#pragma unroll
+    for (int j = 0; j < IS_S; ++j)
+      if (j < L) v[j] += gs[L - 1 - j] * t0 + gs[IS_S + L - 1 - j] * t1 + gs[2 * IS_S + L - 1 - j] * t2;
+  }
+  const double sc = p.scale;
+  if (ok) {
+#pragma unroll
+    for (int j = 0; j < IS_S; ++j)
+      if (j < L) p.dst[base + (int64_t)j * st] = v[j] * sc;
+  }
+}
+
 // Segmentation of an axis of n elements: nw waves of seg elements (the last
 // one shorter, at least 3); false if n is too long for one workgroup.
 static bool iir_segments(int n, int *seg, int *nw) {
@@ -531,9 +675,13 @@ hipError_t launch_iir(const Dense4 &in, Dense4 &out, int axis, const IirCoef &c,
       if (in.dtype == IFM_F32) hipLaunchKernelGGL((k_iir_seg<IFM_F32, true>), dim3(nb), blk, lds, st, p, tb);
       else hipLaunchKernelGGL((k_iir_seg<IFM_F64, true>), dim3(nb), blk, lds, st, p, tb);
     } else {
-      if (in.dtype == IFM_F32) hipLaunchKernelGGL((k_iir_seg<IFM_F32, false>), dim3(nb), blk, lds, st, p, tb);
-      else if (in.dtype == IFM_F64) hipLaunchKernelGGL((k_iir_seg<IFM_F64, false>), dim3(nb), blk, lds, st, p, tb);
-      else hipLaunchKernelGGL((k_iir_seg<-1, false>), dim3(nb), blk, lds, st, p, tb);
+      // two segments per wave, 32 lines per workgroup
+      const unsigned nb2 = (unsigned)((p.lines + IS2_L - 1) / IS2_L);
+      const dim3 blk2(64 * ((nw + 1) / 2));
+      const size_t lds2 = sizeof(double) * iir_seg2_lds_doubles(nw);
+      if (in.dtype == IFM_F32) hipLaunchKernelGGL((k_iir_seg2<IFM_F32>), dim3(nb2), blk2, lds2, st, p, tb);
+      else if (in.dtype == IFM_F64) hipLaunchKernelGGL((k_iir_seg2<IFM_F64>), dim3(nb2), blk2, lds2, st, p, tb);
+      else hipLaunchKernelGGL((k_iir_seg2<-1>), dim3(nb2), blk2, lds2, st, p, tb);
     }
     return hipGetLastError();
   }

@@ -4,11 +4,11 @@
 // (:1027-1048) and rightborder! (:1051-1092): forward recursion with the
 // steady-state left initialisation, right-edge initialisation by the 3x3
 // matrix M (Triggs & Sdika Eqs. 14-15), backward recursion and scaling.
-// Lines along the contiguous axis run k_iir_seg (whole lines in registers,
-// the axis split across waves, an LDS transpose) up to 512 elements and
-// k_iir_rows (LDS-staged chunks) beyond; lines along other axes run
-// k_iir_seg2 (two segments per wave, two workgroups per CU) up to 512
-// elements, k_iir_strided (one thread per line) beyond.  Lanes sit on adjacent lines, which are adjacent in memory for the
+// Lines of up to 512 elements run k_iir_rows2 (contiguous axis, through an
+// LDS transpose) or k_iir_seg2 (strided axes): whole lines in registers, the
+// axis split into segments, two segments per wave.  Longer lines run
+// k_iir_rows (contiguous axis, LDS-staged chunks) or k_iir_strided (one
+// thread per line).  Lanes sit on adjacent lines, which are adjacent in memory for the
 // strided axes (coalesced).
 #include "ifm_internal.h"
 
@@ -255,21 +255,19 @@ __global__ void __launch_bounds__(256, 2) k_iir_rows(IirArgs p) {
   }
 }
 
-// Lines along a strided axis, held in registers.  A workgroup owns 64
-// adjacent lines (lane l: line 64 blockIdx.x + l; adjacent lines are adjacent
-// in memory, so a wave's access to one position is one coalesced row) and
-// cuts the axis into nw segments of at most IS_S elements, one wave each, so
+// Lines of up to 512 elements, held in registers.  The axis is cut into nw
+// segments of at most IS_S elements, each run by one half-wave lane, so
 // every element is read once and written once (the one-thread-per-line form
 // writes the forward pass out and reads it back).  The recursions are linear:
 // a segment's values are its recursion from zero state plus the homogeneous
-// response to the state entering it.  Each wave runs its segment from zero
-// state; the waves exchange their end states through LDS; each composes the
+// response to the state entering it.  Each lane runs its segment from zero
+// state; the lanes exchange their end states through LDS; each composes the
 // state entering its segment from those with the 3x3 transfer matrix of a
 // full segment, and adds sum_c h_c[j] s_c.  The first segment's forward pass
 // and the last segment's backward pass start from the reference's border
 // initialisation directly (the um / M start of k_iir_strided).
 constexpr int IS_S = 64;     // most elements per segment (doubles in registers per lane)
-constexpr int IS_MAXW = 8;   // most segments (waves): n <= 512
+constexpr int IS_MAXW = 8;   // most segments: n <= 512
 
 struct IirSegTables {
   double h[3][IS_S];  // forward homogeneous responses: state (y[-1], y[-2], y[-3]) = e_c
@@ -279,231 +277,61 @@ struct IirSegTables {
 };
 
 // LDS (doubles): tables h, g (6 IS_S), then forward and backward end states
-// [wave][component][lane] (2 x nw x 3 x 64), then (lines along the contiguous
-// axis) one transpose buffer per wave of 32 lines x IS_TP doubles, which also
-// holds 64 lines x IS_TP floats.  At nw = 8 the total is 160,768 bytes.
+// [segment][component][line] (2 x nw x 3 x 32), then (lines along the
+// contiguous axis) one transpose buffer per wave of 16 rows x IS_TP doubles,
+// which also holds 32 rows x IS_TP floats.
 constexpr int IS_TP = IS_S + 1;    // transpose buffer pitch (elements): conflict-free both ways
-constexpr int IS_TB = 32 * IS_TP;  // transpose buffer per wave (doubles)
-__host__ __device__ inline size_t iir_seg_lds_doubles(int nw, bool rows) {
-  return 6 * IS_S + 2 * (size_t)nw * 3 * 64 + (rows ? (size_t)nw * IS_TB : 0);
+
+// Two segments per wave: a workgroup owns 32 adjacent lines and lane l of
+// wave w runs segment 2w + (l >> 5) of line l & 31.  A workgroup then needs
+// half the waves of one segment per wave, so two workgroups share a CU (each
+// wave still holds a whole segment in registers) and one's loads and stores
+// overlap the other's recursions.
+//  * k_iir_seg2  strided lines: each half-wave reads one contiguous run of 32
+//                lines per element.
+//  * k_iir_rows2 contiguous lines: the segments pass through a per-wave LDS
+//                transpose (rows = (line, half) pairs, lanes along the
+//                segment for the global loads and stores), 16 lines per
+//                round for FP32 input, 8 for FP64 input and for the output.
+constexpr int IS2_L = 32;                 // lines per workgroup
+constexpr int IS2_TB = 16 * IS_TP;        // transpose buffer per wave (doubles)
+__host__ __device__ inline size_t iir_seg2_lds_doubles(int nw, bool rows) {
+  return 6 * IS_S + 2 * (size_t)nw * 3 * IS2_L + (rows ? (size_t)((nw + 1) / 2) * IS2_TB : 0);
 }
 
-// ROWS: lines along the contiguous axis (F32 or F64 input).  A line's
-// segment is then IS_S consecutive elements, so a wave moves it through its
-// transpose buffer: lanes run along the segment for the global loads and
-// stores (one contiguous run of a line per instruction), and along the lines
-// for the recursion.  F32 input moves all 64 lines in one round (float
-// buffer), F64 input and the F64 output in two rounds of 32 lines.
-template <int DT, bool ROWS>
-__global__ void __launch_bounds__(64 * IS_MAXW) k_iir_seg(IirArgs p, IirSegTables tb) {
-  extern __shared__ double sm[];
-  double *hs = sm, *gs = sm + 3 * IS_S;
-  double *stf = sm + 6 * IS_S, *stb = stf + tb.nw * 3 * 64;
+struct Seg2 {
+  int ln, sg, L;
+  bool has, first, last;
+};
+
+__device__ __forceinline__ Seg2 seg2_lane(const IirSegTables &tb, int n) {
+  Seg2 g;
   const int lane = threadIdx.x & 63;
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int nw = tb.nw, S = tb.seg;
-  const int n = p.n[p.axis];
-  const int s0 = w * S;
-  const int L = min(S, n - s0);
-  const bool first = w == 0, last = w == nw - 1;
-  const int64_t line = (int64_t)blockIdx.x * 64 + lane;
-  const bool ok = line < p.lines;
-  const int64_t st = p.stride;
-  const int64_t base = (line % st) + (line / st) * st * n + (int64_t)s0 * st;
-  double v[IS_S];
-  double *tr = sm + iir_seg_lds_doubles(nw, false) + w * IS_TB;  // ROWS only
-  if constexpr (ROWS) {
-    static_assert(DT == IFM_F32 || DT == IFM_F64, "typed contiguous lines only");
-    using TE = typename std::conditional<DT == IFM_F32, float, double>::type;
-    constexpr int TRL = DT == IFM_F32 ? 64 : 32;  // lines per round
-    TE *trl = (TE *)tr;
-    const TE *src = (const TE *)p.src;
-    const int64_t line0 = (int64_t)blockIdx.x * 64;
-    for (int r = 0; r < 64 / TRL; ++r) {
-      TE x[TRL];
-#pragma unroll
-      for (int i = 0; i < TRL; ++i) {
-        const int64_t ln = line0 + r * TRL + i;
-        x[i] = (lane < L && ln < p.lines) ? src[ln * n + s0 + lane] : (TE)0;
-      }
-#pragma unroll
-      for (int i = 0; i < TRL; ++i) trl[i * IS_TP + lane] = x[i];
-      __syncthreads();
-      if (lane / TRL == r) {
-#pragma unroll
-        for (int j = 0; j < IS_S; ++j)
-          if (j < L) v[j] = (double)trl[(lane % TRL) * IS_TP + j];
-      }
-      __syncthreads();
-    }
-  } else {
-#pragma unroll
-    for (int j = 0; j < IS_S; ++j)
-      if (j < L && ok) v[j] = iir_ld<DT>(p, base + (int64_t)j * st);
-  }
-  for (int i = threadIdx.x; i < 6 * IS_S; i += blockDim.x) sm[i] = i < 3 * IS_S ? tb.h[i / IS_S][i % IS_S]
-                                                                                   : tb.g[i / IS_S - 3][i % IS_S];
-  const double a1 = p.a[0], a2 = p.a[1], a3 = p.a[2];
-  const double b1 = p.b[0], b2 = p.b[1], b3 = p.b[2];
-  const double asum = a1 + a2 + a3, bsum = b1 + b2 + b3;
-  // forward: the first segment from y[-k] = u- (leftborder!), the others from zero
-  double ym1 = 0.0, ym2 = 0.0, ym3 = 0.0, xlast = 0.0;
-  if (first) {
-    const double iminus = p.isfill ? p.fill : v[0];
-    ym1 = ym2 = ym3 = iminus / (1 - asum);
-  }
-#pragma unroll
-  for (int j = 0; j < IS_S; ++j) {
-    if (j < L) {
-      if (j == L - 1) xlast = v[j];
-      const double y = v[j] + a1 * ym1 + a2 * ym2 + a3 * ym3;
-      v[j] = y;
-      ym3 = ym2;
-      ym2 = ym1;
-      ym1 = y;
-    }
-  }
-  stf[(w * 3 + 0) * 64 + lane] = ym1;
-  stf[(w * 3 + 1) * 64 + lane] = ym2;
-  stf[(w * 3 + 2) * 64 + lane] = ym3;
-  __syncthreads();
-  // state entering segment w: T_0 = end of segment 0, T_u = Phi T_{u-1} + P_u
-  // with Phi[r][c] = h_c[S - 1 - r]
-  double e0 = ym1, e1 = ym2, e2 = ym3;  // true end state of this segment
-  if (!first) {
-    double t0 = stf[lane], t1 = stf[64 + lane], t2 = stf[128 + lane];
-    for (int u = 1; u < w; ++u) {
-      const double *q = stf + u * 192;
-      const double n0 = hs[S - 1] * t0 + hs[IS_S + S - 1] * t1 + hs[2 * IS_S + S - 1] * t2 + q[lane];
-      const double n1 = hs[S - 2] * t0 + hs[IS_S + S - 2] * t1 + hs[2 * IS_S + S - 2] * t2 + q[64 + lane];
-      const double n2 = hs[S - 3] * t0 + hs[IS_S + S - 3] * t1 + hs[2 * IS_S + S - 3] * t2 + q[128 + lane];
-      t0 = n0;
-      t1 = n1;
-      t2 = n2;
-    }
-#pragma unroll
-    for (int j = 0; j < IS_S; ++j)
-      if (j < L) v[j] += hs[j] * t0 + hs[IS_S + j] * t1 + hs[2 * IS_S + j] * t2;
-    e0 = ym1 + hs[L - 1] * t0 + hs[IS_S + L - 1] * t1 + hs[2 * IS_S + L - 1] * t2;
-    e1 = ym2 + hs[L - 2] * t0 + hs[IS_S + L - 2] * t1 + hs[2 * IS_S + L - 2] * t2;
-    e2 = ym3 + hs[L - 3] * t0 + hs[IS_S + L - 3] * t1 + hs[2 * IS_S + L - 3] * t2;
-  }
-  // backward: the last segment from the rightborder! values (e = y[n-1],
-  // y[n-2], y[n-3]), the others from zero
-  double z0 = 0.0, z1 = 0.0, z2 = 0.0;
-  if (last) {
-    const double iplus = p.isfill ? p.fill : xlast;
-    const double uplus = iplus / (1 - asum);
-    const double vplus = uplus / (1 - bsum);
-    const double d0 = e0 - uplus, d1 = e1 - uplus, d2 = e2 - uplus;
-    const double m0 = p.M[0] * d0 + p.M[1] * d1 + p.M[2] * d2 + vplus;
-    const double m1 = p.M[3] * d0 + p.M[4] * d1 + p.M[5] * d2 + vplus;
-    const double m2 = p.M[6] * d0 + p.M[7] * d1 + p.M[8] * d2 + vplus;
-    z2 = m0;
-    z1 = e1 + b1 * z2 + b2 * m1 + b3 * m2;
-    z0 = e2 + b1 * z1 + b2 * z2 + b3 * m1;
-  }
-  double zp1 = 0.0, zp2 = 0.0, zp3 = 0.0;
-#pragma unroll
-  for (int j = IS_S - 1; j >= 0; --j) {
-    if (j < L) {
-      double z;
-      if (last && j == L - 1) z = z2;
-      else if (last && j == L - 2) z = z1;
-      else if (last && j == L - 3) z = z0;
-      else z = v[j] + b1 * zp1 + b2 * zp2 + b3 * zp3;
-      v[j] = z;
-      zp3 = zp2;
-      zp2 = zp1;
-      zp1 = z;
-    }
-  }
-  stb[(w * 3 + 0) * 64 + lane] = zp1;
-  stb[(w * 3 + 1) * 64 + lane] = zp2;
-  stb[(w * 3 + 2) * 64 + lane] = zp3;
-  __syncthreads();
-  // state entering segment w from the right: R_{nw-1} = left end of the last
-  // segment, R_u = Psi R_{u+1} + Q_u with Psi[r][c] = g_c[S - 1 - r]
-  if (!last) {
-    const double *q0 = stb + (nw - 1) * 192;
-    double t0 = q0[lane], t1 = q0[64 + lane], t2 = q0[128 + lane];
-    for (int u = nw - 2; u > w; --u) {
-      const double *q = stb + u * 192;
-      const double n0 = gs[S - 1] * t0 + gs[IS_S + S - 1] * t1 + gs[2 * IS_S + S - 1] * t2 + q[lane];
-      const double n1 = gs[S - 2] * t0 + gs[IS_S + S - 2] * t1 + gs[2 * IS_S + S - 2] * t2 + q[64 + lane];
-      const double n2 = gs[S - 3] * t0 + gs[IS_S + S - 3] * t1 + gs[2 * IS_S + S - 3] * t2 + q[128 + lane];
-      t0 = n0;
-      t1 = n1;
-      t2 = n2;
-    }
-#pragma unroll
-    for (int j = 0; j < IS_S; ++j)
-      if (j < L) v[j] += gs[L - 1 - j] * t0 + gs[IS_S + L - 1 - j] * t1 + gs[2 * IS_S + L - 1 - j] * t2;
-  }
-  const double sc = p.scale;
-  if constexpr (ROWS) {
-    const int64_t line0 = (int64_t)blockIdx.x * 64;
-    for (int r = 0; r < 2; ++r) {
-      if ((lane >> 5) == r) {
-#pragma unroll
-        for (int j = 0; j < IS_S; ++j)
-          if (j < L) tr[(lane & 31) * IS_TP + j] = v[j] * sc;
-      }
-      __syncthreads();
-#pragma unroll
-      for (int i = 0; i < 32; ++i) {
-        const int64_t ln = line0 + r * 32 + i;
-        if (lane < L && ln < p.lines) p.dst[ln * n + s0 + lane] = tr[i * IS_TP + lane];
-      }
-      __syncthreads();
-    }
-  } else if (ok) {
-#pragma unroll
-    for (int j = 0; j < IS_S; ++j)
-      if (j < L) p.dst[base + (int64_t)j * st] = v[j] * sc;
-  }
+  g.ln = lane & (IS2_L - 1);
+  g.sg = 2 * (int)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) + (lane >> 5);
+  g.has = g.sg < tb.nw;
+  g.L = g.has ? min(tb.seg, n - g.sg * tb.seg) : 0;
+  g.first = g.sg == 0;
+  g.last = g.sg == tb.nw - 1;
+  return g;
 }
 
-// Strided lines, two segments per wave: the same segmented recursion as
-// k_iir_seg, but a workgroup owns 32 adjacent lines and lane l of wave w runs
-// segment 2w + (l >> 5) of line l & 31.  A workgroup then needs half the
-// waves for the same axis, so two workgroups share a CU (each wave still
-// holds a whole segment in registers) and one's loads and stores overlap the
-// other's recursions.  Each half-wave reads one contiguous run of 32 lines.
-constexpr int IS2_L = 32;  // lines per workgroup
-__host__ __device__ inline size_t iir_seg2_lds_doubles(int nw) { return 6 * IS_S + 2 * (size_t)nw * 3 * IS2_L; }
-
-template <int DT>
-__global__ void __launch_bounds__(32 * IS_MAXW) k_iir_seg2(IirArgs p, IirSegTables tb) {
-  extern __shared__ double sm[];
+// The segmented recursion on v[0 .. L) of every lane: forward from zero state (the first segment from leftborder!),
+// end states exchanged through LDS, the homogeneous response of the state
+// entering the segment added; the same backward (the last segment from
+// rightborder!).  Leaves v scaled by nothing; contains two barriers.
+__device__ __forceinline__ void seg2_filter(const IirArgs &p, const IirSegTables &tb, const Seg2 &g, double *sm,
+                                            double (&v)[IS_S]) {
   double *hs = sm, *gs = sm + 3 * IS_S;
   double *stf = sm + 6 * IS_S, *stb = stf + tb.nw * 3 * IS2_L;
-  const int lane = threadIdx.x & 63;
-  const int ln = lane & (IS2_L - 1);
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int nw = tb.nw, S = tb.seg;
-  const int sg = 2 * w + (lane >> 5);  // segment of this lane (>= nw: none)
-  const int n = p.n[p.axis];
-  const int s0 = sg * S;
-  const bool has = sg < nw;
-  const int L = has ? min(S, n - s0) : 0;
-  const bool first = sg == 0, last = sg == nw - 1;
-  const int64_t line = (int64_t)blockIdx.x * IS2_L + ln;
-  const bool ok = line < p.lines && has;
-  const int64_t st = p.stride;
-  const int64_t base = (line % st) + (line / st) * st * n + (int64_t)s0 * st;
-  double v[IS_S];
-#pragma unroll
-  for (int j = 0; j < IS_S; ++j)
-    if (j < L && ok) v[j] = iir_ld<DT>(p, base + (int64_t)j * st);
+  const int nw = tb.nw, S = tb.seg, L = g.L, ln = g.ln, sg = g.sg;
   for (int i = threadIdx.x; i < 6 * IS_S; i += blockDim.x) sm[i] = i < 3 * IS_S ? tb.h[i / IS_S][i % IS_S]
                                                                                    : tb.g[i / IS_S - 3][i % IS_S];
   const double a1 = p.a[0], a2 = p.a[1], a3 = p.a[2];
   const double b1 = p.b[0], b2 = p.b[1], b3 = p.b[2];
   const double asum = a1 + a2 + a3, bsum = b1 + b2 + b3;
-  // forward: the first segment from y[-k] = u- (leftborder!), the others from zero
   double ym1 = 0.0, ym2 = 0.0, ym3 = 0.0, xlast = 0.0;
-  if (first) {
+  if (g.first) {
     const double iminus = p.isfill ? p.fill : v[0];
     ym1 = ym2 = ym3 = iminus / (1 - asum);
   }
@@ -518,7 +346,7 @@ __global__ void __launch_bounds__(32 * IS_MAXW) k_iir_seg2(IirArgs p, IirSegTabl
       ym1 = y;
     }
   }
-  if (has) {
+  if (g.has) {
     stf[(sg * 3 + 0) * IS2_L + ln] = ym1;
     stf[(sg * 3 + 1) * IS2_L + ln] = ym2;
     stf[(sg * 3 + 2) * IS2_L + ln] = ym3;
@@ -527,7 +355,7 @@ __global__ void __launch_bounds__(32 * IS_MAXW) k_iir_seg2(IirArgs p, IirSegTabl
   // state entering segment sg: T_0 = end of segment 0, T_u = Phi T_{u-1} + P_u
   // with Phi[r][c] = h_c[S - 1 - r]
   double e0 = ym1, e1 = ym2, e2 = ym3;  // true end state of this segment
-  if (has && !first) {
+  if (g.has && !g.first) {
     double t0 = stf[ln], t1 = stf[IS2_L + ln], t2 = stf[2 * IS2_L + ln];
     for (int u = 1; u < sg; ++u) {
       const double *q = stf + u * 3 * IS2_L;
@@ -545,10 +373,8 @@ __global__ void __launch_bounds__(32 * IS_MAXW) k_iir_seg2(IirArgs p, IirSegTabl
     e1 = ym2 + hs[L - 2] * t0 + hs[IS_S + L - 2] * t1 + hs[2 * IS_S + L - 2] * t2;
     e2 = ym3 + hs[L - 3] * t0 + hs[IS_S + L - 3] * t1 + hs[2 * IS_S + L - 3] * t2;
   }
-  // backward: the last segment from the rightborder! values (e = y[n-1],
-  // y[n-2], y[n-3]), the others from zero
   double z0 = 0.0, z1 = 0.0, z2 = 0.0;
-  if (last) {
+  if (g.last) {
     const double iplus = p.isfill ? p.fill : xlast;
     const double uplus = iplus / (1 - asum);
     const double vplus = uplus / (1 - bsum);
@@ -565,9 +391,9 @@ __global__ void __launch_bounds__(32 * IS_MAXW) k_iir_seg2(IirArgs p, IirSegTabl
   for (int j = IS_S - 1; j >= 0; --j) {
     if (j < L) {
       double z;
-      if (last && j == L - 1) z = z2;
-      else if (last && j == L - 2) z = z1;
-      else if (last && j == L - 3) z = z0;
+      if (g.last && j == L - 1) z = z2;
+      else if (g.last && j == L - 2) z = z1;
+      else if (g.last && j == L - 3) z = z0;
       else z = v[j] + b1 * zp1 + b2 * zp2 + b3 * zp3;
       v[j] = z;
       zp3 = zp2;
@@ -575,7 +401,7 @@ __global__ void __launch_bounds__(32 * IS_MAXW) k_iir_seg2(IirArgs p, IirSegTabl
       zp1 = z;
     }
   }
-  if (has) {
+  if (g.has) {
     stb[(sg * 3 + 0) * IS2_L + ln] = zp1;
     stb[(sg * 3 + 1) * IS2_L + ln] = zp2;
     stb[(sg * 3 + 2) * IS2_L + ln] = zp3;
@@ -583,7 +409,7 @@ __global__ void __launch_bounds__(32 * IS_MAXW) k_iir_seg2(IirArgs p, IirSegTabl
   __syncthreads();
   // state entering segment sg from the right: R_{nw-1} = left end of the last
   // segment, R_u = Psi R_{u+1} + Q_u with Psi[r][c] = g_c[S - 1 - r]
-  if (has && !last) {
+  if (g.has && !g.last) {
     const double *q0 = stb + (nw - 1) * 3 * IS2_L;
     double t0 = q0[ln], t1 = q0[IS2_L + ln], t2 = q0[2 * IS2_L + ln];
     for (int u = nw - 2; u > sg; --u) {
@@ -599,11 +425,87 @@ __global__ void __launch_bounds__(32 * IS_MAXW) k_iir_seg2(IirArgs p, IirSegTabl
     for (int j = 0; j < IS_S; ++j)
       if (j < L) v[j] += gs[L - 1 - j] * t0 + gs[IS_S + L - 1 - j] * t1 + gs[2 * IS_S + L - 1 - j] * t2;
   }
+}
+
+template <int DT>
+__global__ void __launch_bounds__(32 * IS_MAXW) k_iir_seg2(IirArgs p, IirSegTables tb) {
+  extern __shared__ double sm[];
+  const int n = p.n[p.axis];
+  const Seg2 g = seg2_lane(tb, n);
+  const int64_t line = (int64_t)blockIdx.x * IS2_L + g.ln;
+  const bool ok = line < p.lines && g.has;
+  const int64_t st = p.stride;
+  const int64_t base = (line % st) + (line / st) * st * n + (int64_t)g.sg * tb.seg * st;
+  double v[IS_S];
+#pragma unroll
+  for (int j = 0; j < IS_S; ++j)
+    if (j < g.L && ok) v[j] = iir_ld<DT>(p, base + (int64_t)j * st);
+  seg2_filter(p, tb, g, sm, v);
   const double sc = p.scale;
   if (ok) {
 #pragma unroll
     for (int j = 0; j < IS_S; ++j)
-      if (j < L) p.dst[base + (int64_t)j * st] = v[j] * sc;
+      if (j < g.L) p.dst[base + (int64_t)j * st] = v[j] * sc;
+  }
+}
+
+template <int DT>
+__global__ void __launch_bounds__(32 * IS_MAXW) k_iir_rows2(IirArgs p, IirSegTables tb) {
+  static_assert(DT == IFM_F32 || DT == IFM_F64, "typed contiguous lines only");
+  using TE = typename std::conditional<DT == IFM_F32, float, double>::type;
+  extern __shared__ double sm[];
+  const int n = p.n[p.axis];
+  const Seg2 g = seg2_lane(tb, n);
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63, half = lane >> 5;
+  const int S = tb.seg, nw = tb.nw;
+  const int64_t line0 = (int64_t)blockIdx.x * IS2_L;
+  double *tr = sm + iir_seg2_lds_doubles(nw, false) + w * IS2_TB;
+  const TE *src = (const TE *)p.src;
+  double v[IS_S];
+  // loads: row i of a round = line r * RI + i / 2, segment 2w + i % 2; lane = element
+  constexpr int RI = DT == IFM_F32 ? 16 : 8;
+  TE *tri = (TE *)tr;
+  for (int r = 0; r < IS2_L / RI; ++r) {
+    TE x[2 * RI];
+#pragma unroll
+    for (int i = 0; i < 2 * RI; ++i) {
+      const int sgi = 2 * w + (i & 1);
+      const int Li = sgi < nw ? min(S, n - sgi * S) : 0;
+      const int64_t ln = line0 + r * RI + (i >> 1);
+      x[i] = (lane < Li && ln < p.lines) ? src[ln * n + sgi * S + lane] : (TE)0;
+    }
+#pragma unroll
+    for (int i = 0; i < 2 * RI; ++i) tri[i * IS_TP + lane] = x[i];
+    __syncthreads();
+    if ((g.ln / RI) == r) {
+      const int row = (g.ln - r * RI) * 2 + half;
+#pragma unroll
+      for (int j = 0; j < IS_S; ++j)
+        if (j < g.L) v[j] = (double)tri[row * IS_TP + j];
+    }
+    __syncthreads();
+  }
+  seg2_filter(p, tb, g, sm, v);
+  const double sc = p.scale;
+  // stores: rounds of 8 lines (16 rows of doubles)
+  constexpr int RO = 8;
+  for (int r = 0; r < IS2_L / RO; ++r) {
+    if ((g.ln / RO) == r) {
+      const int row = (g.ln - r * RO) * 2 + half;
+#pragma unroll
+      for (int j = 0; j < IS_S; ++j)
+        if (j < g.L) tr[row * IS_TP + j] = v[j] * sc;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 2 * RO; ++i) {
+      const int sgi = 2 * w + (i & 1);
+      const int Li = sgi < nw ? min(S, n - sgi * S) : 0;
+      const int64_t ln = line0 + r * RO + (i >> 1);
+      if (lane < Li && ln < p.lines) p.dst[ln * n + sgi * S + lane] = tr[i * IS_TP + lane];
+    }
+    __syncthreads();
   }
 }
 
@@ -667,18 +569,15 @@ hipError_t launch_iir(const Dense4 &in, Dense4 &out, int axis, const IirCoef &c,
         gm[0] = g;
       }
     }
+    // two segments per wave, 32 lines per workgroup
     const bool rows = stride == 1;
-    const unsigned nb = (unsigned)((p.lines + 63) / 64);
-    const size_t lds = sizeof(double) * iir_seg_lds_doubles(nw, rows);
-    const dim3 blk(64 * nw);
+    const unsigned nb2 = (unsigned)((p.lines + IS2_L - 1) / IS2_L);
+    const dim3 blk2(64 * ((nw + 1) / 2));
+    const size_t lds2 = sizeof(double) * iir_seg2_lds_doubles(nw, rows);
     if (rows) {
-      if (in.dtype == IFM_F32) hipLaunchKernelGGL((k_iir_seg<IFM_F32, true>), dim3(nb), blk, lds, st, p, tb);
-      else hipLaunchKernelGGL((k_iir_seg<IFM_F64, true>), dim3(nb), blk, lds, st, p, tb);
+      if (in.dtype == IFM_F32) hipLaunchKernelGGL((k_iir_rows2<IFM_F32>), dim3(nb2), blk2, lds2, st, p, tb);
+      else hipLaunchKernelGGL((k_iir_rows2<IFM_F64>), dim3(nb2), blk2, lds2, st, p, tb);
     } else {
-      // two segments per wave, 32 lines per workgroup
-      const unsigned nb2 = (unsigned)((p.lines + IS2_L - 1) / IS2_L);
-      const dim3 blk2(64 * ((nw + 1) / 2));
-      const size_t lds2 = sizeof(double) * iir_seg2_lds_doubles(nw);
       if (in.dtype == IFM_F32) hipLaunchKernelGGL((k_iir_seg2<IFM_F32>), dim3(nb2), blk2, lds2, st, p, tb);
       else if (in.dtype == IFM_F64) hipLaunchKernelGGL((k_iir_seg2<IFM_F64>), dim3(nb2), blk2, lds2, st, p, tb);
       else hipLaunchKernelGGL((k_iir_seg2<-1>), dim3(nb2), blk2, lds2, st, p, tb);

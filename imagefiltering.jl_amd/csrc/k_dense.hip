@@ -6,7 +6,7 @@
 //
 //  * k_dense_nd     any rank <= 4, one output per thread (small kernels, 3-D)
 //  * k_dense2d_valu 2-D, LDS tile, 8 rows per thread, FP32 FMA
-//  * k_dense2d_small the same, register-blocked for odd kernel sizes 3 .. 13
+//  * k_dense2d_small the same, register-blocked for odd kernel sizes 3 .. 15
 // The matrix-core path for 2-D kernels is k_mfma.hip.
 #include "ifm_internal.h"
 
@@ -135,7 +135,7 @@ __global__ void __launch_bounds__(256) k_dense2d_valu(Dense2dArgs a) {
 
 // ---------------------------------------------------------------------------
 // 2-D VALU, register-blocked for compile-time kernel sizes KX x KY (odd, 3 ..
-// 13): the same 64 x 32 tile and thread shape as k_dense2d_valu, but every
+// 15): the same 64 x 32 tile and thread shape as k_dense2d_valu, but every
 // loop is unrolled, the taps arrive as kernel arguments (scalar registers,
 // v_fmac with an SGPR operand), and each LDS value read feeds up to KY * 8 /
 // (KY + 7) FMAs through a sliding column of KX values.
@@ -267,11 +267,12 @@ static hipError_t launch_small_ky(int ky, const Dense2dArgs &a, const float *tap
   case 7: return launch_small<KX, 7>(a, taps, grid, st);
   case 9: return launch_small<KX, 9>(a, taps, grid, st);
   case 11: return launch_small<KX, 11>(a, taps, grid, st);
-  default: return launch_small<KX, 13>(a, taps, grid, st);
+  case 13: return launch_small<KX, 13>(a, taps, grid, st);
+  default: return launch_small<KX, 15>(a, taps, grid, st);
   }
 }
 
-static bool small_size(int n) { return n >= 3 && n <= 13 && (n & 1); }
+static bool small_size(int n) { return n >= 3 && n <= 15 && (n & 1); }
 
 hipError_t launch_dense_valu(const Dense4 &in, Dense4 &out, const int64_t *out_first, const DenseKernel &k,
                              const float *d_taps, int style, float fill, hipStream_t st) {
@@ -315,7 +316,8 @@ hipError_t launch_dense_valu(const Dense4 &in, Dense4 &out, const int64_t *out_f
       case 7: return launch_small_ky<7>(a.k1, a, taps, grid, st);
       case 9: return launch_small_ky<9>(a.k1, a, taps, grid, st);
       case 11: return launch_small_ky<11>(a.k1, a, taps, grid, st);
-      default: return launch_small_ky<13>(a.k1, a, taps, grid, st);
+      case 13: return launch_small_ky<13>(a.k1, a, taps, grid, st);
+      default: return launch_small_ky<15>(a.k1, a, taps, grid, st);
       }
     }
     hipLaunchKernelGGL(k_dense2d_valu, grid, dim3(256), lds, st, a);

@@ -249,9 +249,10 @@ def test_dense_valu_small_kernels(ks):
         assert relerr(out, want) <= TOL, (ks, b)
 
 
-@pytest.mark.parametrize("ks", [(3, 3), (5, 5), (7, 7), (9, 13), (13, 11), (3, 13), (11, 5), (13, 13)])
+@pytest.mark.parametrize("ks", [(3, 3), (5, 5), (7, 7), (9, 13), (13, 11), (3, 13), (11, 5), (13, 13), (15, 15),
+                                (15, 7), (3, 15)])
 def test_dense_register_blocked_sizes(ks):
-    """The register-blocked VALU kernels (odd sizes 3 .. 13 per axis):
+    """The register-blocked VALU kernels (odd sizes 3 .. 15 per axis):
     kernels off centre, images with interior and border tiles, a 3-D image
     (planes), F64 / BF16 / UInt8 inputs, every border."""
     from imagefiltering import _lib

@@ -524,6 +524,17 @@ def run_extras(IF, _lib, L, ctx, T, img, d_img, d_out, b_reflect, args, cpu, hea
          cpu_baseline=cpu.entry(512 * 512 * 32, tis, 1, "512 x 512 x 32 sub-volume, FP64 Triggs-Sdika (three axes)",
                                 "one thread, as the reference's IIR"),
          cpu_parallel=cpu.entry(nv, tip, cpu.threads, "whole volume", "all cores (parallel restatement)"))
+    # a dense 3-D stencil (register-blocked): imfilter(vol, centered(3x3x3), "reflect", Algorithm.FIR()), F64 out
+    k3 = np.random.default_rng(20261230).standard_normal((3, 3, 3))
+    k3 /= k3.sum()
+    rate("dense3d_3x3x3_512x512x256", api(IF.Algorithm.FIR(), o_vol, d_vol, IF.centered(k3), "reflect"), nv, 12, 3,
+         kernel="k_dense3d_small<3,3,3>")
+    z3 = 120
+    w3, t3, u3, px3 = cpu.scaled(lambda u: (lambda: O.imfilter_dense(vol64, k3, [-1, -1, -1], "reflect", [0, 0, z3],
+                                                                     [512, 512, u]), 512 * 512 * u), 16, 1, 1)
+    note("dense3d_3x3x3_512x512x256", max_rel_err_vs_fp64=relerr(np.asarray(o_vol.to_host())[:, :, z3:z3 + u3], w3),
+         cpu_baseline=cpu.entry(px3, t3, 1, "512 x 512 x %d planes, direct FP64 FIR" % u3,
+                                "one thread, as CPU1(FIR())"))
     del d_vol, o_vol, vol64, wi, sub
 
     # 8192^2 N0f8 with 5% salt and pepper: median 5x5 / 7x7, extrema 3x3 / 9x9, minimum 9x9

@@ -4,7 +4,8 @@
 // sum_J A[i+J] k[J] on the padded image.  The border map is applied while the
 // input tile is staged into LDS.
 //
-//  * k_dense_nd     any rank <= 4, one output per thread (small kernels, 3-D)
+//  * k_dense_nd     any rank <= 4, one output per thread (other kernels)
+//  * k_dense3d_small 3-D kernels of 3 or 5 taps per axis, register-blocked
 //  * k_dense2d_valu 2-D, LDS tile, 8 rows per thread, FP32 FMA
 //  * k_dense2d_small the same, register-blocked for odd kernel sizes 3 .. 15
 // The matrix-core path for 2-D kernels is k_mfma.hip.
@@ -274,6 +275,203 @@ static hipError_t launch_small_ky(int ky, const Dense2dArgs &a, const float *tap
 
 static bool small_size(int n) { return n >= 3 && n <= 15 && (n & 1); }
 
+// ---------------------------------------------------------------------------
+// 3-D VALU, register-blocked for compile-time kernel sizes KX x KY x KZ (3 or
+// 5 per axis): a workgroup owns a 64 x 32 (x, y) tile and marches over D3_NZ
+// output planes.  The KZ input planes an output plane needs sit in an LDS
+// ring of staged planes (the k_dense2d_small layout); each step commits the
+// plane its loads brought in, issues the loads of the next one, and sums the
+// KZ plane slices with their kernel slices (taps in scalar registers), so
+// every input plane is read from HBM once per workgroup.
+// ---------------------------------------------------------------------------
+constexpr int D3_NZ = 32;  // output planes per workgroup
+
+struct Dense3dArgs {
+  const void *in;
+  void *out;
+  int n0, n1, n2, n3;           // input extents
+  int o0, o1, o2;               // output extents (axis 3 through the grid)
+  int g0, g1, g2, g3;           // image index of output (0, 0, 0, 0)'s window start
+  int64_t in_plane, out_plane;  // elements per (x, y) plane
+  int nzc;                      // z chunks
+  int style;
+  int dt_in, dt_out;
+  float fill;
+};
+
+template <int KX, int KY, int KZ>
+struct SmallTaps3 {
+  float k[KX * KY * KZ];  // column-major (axis 0 fastest)
+};
+
+template <int KX, int KY, int KZ>
+__global__ void __launch_bounds__(256) k_dense3d_small(Dense3dArgs a, SmallTaps3<KX, KY, KZ> kt) {
+  constexpr int IN_W = DV_TX + KX - 1, IN_H = DV_TY + KY - 1, IN_WP = IN_W | 1;
+  constexpr int PL = IN_H * IN_WP;  // floats per staged plane
+  constexpr int NE = IN_H * IN_W, NI = (NE + 255) / 256;
+  __shared__ float smem[KZ * PL];
+  __shared__ int smap[IN_H + IN_W];
+  const int tid = threadIdx.x;
+  const int tx = tid & 63, ty = tid >> 6;
+  const int nxy = gridDim.x * gridDim.y;
+  const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y + nxy * blockIdx.z, nxy * gridDim.z);
+  const int bz = tile / nxy, bx = tile % gridDim.x, by = (tile % nxy) / gridDim.x;
+  const int zc = bz % a.nzc, p3 = bz / a.nzc;
+  const int oz0 = zc * D3_NZ, nz = min(D3_NZ, a.o2 - oz0);
+  const int gx = a.g0 + bx * DV_TX, gy = a.g1 + by * DV_TY;
+  const bool interior = gx >= 0 && gx + IN_W <= a.n0 && gy >= 0 && gy + IN_H <= a.n1;
+  if (!interior) {
+    for (int i = tid; i < IN_H + IN_W; i += 256)
+      smap[i] = i < IN_H ? bmap(a.style, gy + i, a.n1) : bmap(a.style, gx + i - IN_H, a.n0);
+    __syncthreads();
+  }
+  const int s3 = bmap(a.style, p3 + a.g3, a.n3);
+  const int64_t vbase = s3 < 0 ? 0 : (int64_t)s3 * a.n2 * a.in_plane;
+  const bool f32 = a.dt_in == IFM_F32;
+  // input plane of relative index j (-1: a Fill plane)
+  auto zin = [&](int j) {
+    const int z = bmap(a.style, a.g2 + oz0 + j, a.n2);
+    return s3 < 0 ? -1 : z;
+  };
+  float x[NI];
+  auto fetch = [&](int zi) {  // FP32: the plane's loads into x
+    if (!f32) return;
+    const float *src = (const float *)a.in + vbase + (int64_t)(zi < 0 ? 0 : zi) * a.in_plane;
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+      const int e = tid + 256 * i;
+      const int r = e / IN_W, c = e - r * IN_W;
+      if (e < NE) {
+        if (zi < 0) {
+          x[i] = a.fill;
+        } else if (interior) {
+          x[i] = src[(int64_t)(gy + r) * a.n0 + gx + c];
+        } else {
+          const int sy = smap[r], sx = smap[IN_H + c];
+          x[i] = (sy < 0 || sx < 0) ? a.fill : src[(int64_t)sy * a.n0 + sx];
+        }
+      }
+    }
+  };
+  auto commit = [&](int zi, int slot) {
+    float *dst = smem + slot * PL;
+    if (f32) {
+#pragma unroll
+      for (int i = 0; i < NI; ++i) {
+        const int e = tid + 256 * i;
+        const int r = e / IN_W, c = e - r * IN_W;
+        if (e < NE) dst[r * IN_WP + c] = x[i];
+      }
+    } else {
+      for (int e = tid; e < NE; e += 256) {
+        const int r = e / IN_W, c = e - r * IN_W;
+        const int sy = interior ? gy + r : smap[r], sx = interior ? gx + c : smap[IN_H + c];
+        dst[r * IN_WP + c] = (zi < 0 || sy < 0 || sx < 0)
+                                 ? a.fill
+                                 : load_f(a.in, vbase + (int64_t)zi * a.in_plane + (int64_t)sy * a.n0 + sx, a.dt_in);
+      }
+    }
+  };
+  // prologue: input planes 0 .. KZ-2 staged, plane KZ-1 in flight
+  for (int j = 0; j < KZ - 1; ++j) {
+    const int zi = zin(j);
+    fetch(zi);
+    commit(zi, j);
+  }
+  int zn = zin(KZ - 1);
+  fetch(zn);
+  const int ox = bx * DV_TX + tx;
+  for (int q = 0; q < nz; ++q) {
+    commit(zn, (q + KZ - 1) % KZ);
+    __syncthreads();  // input planes q .. q + KZ - 1 staged
+    if (q + 1 < nz) {
+      zn = zin(q + KZ);
+      fetch(zn);
+    }
+    float acc[DV_R];
+#pragma unroll
+    for (int o = 0; o < DV_R; ++o) acc[o] = 0.f;
+#pragma unroll
+    for (int w = 0; w < KZ; ++w) {
+      const float *base = smem + ((q + w) % KZ) * PL + ty * DV_R * IN_WP + tx;
+#pragma unroll
+      for (int u = 0; u < KY + DV_R - 1; ++u) {
+        float v[KX];
+#pragma unroll
+        for (int t = 0; t < KX; ++t) v[t] = base[u * IN_WP + t];
+#pragma unroll
+        for (int o = 0; o < DV_R; ++o) {
+          const int j = u - o;
+          if (j >= 0 && j < KY) {
+#pragma unroll
+            for (int t = 0; t < KX; ++t) acc[o] = fmaf(kt.k[t + j * KX + w * KX * KY], v[t], acc[o]);
+          }
+        }
+      }
+    }
+    if (ox < a.o0) {
+      const int64_t obase = ((int64_t)p3 * a.o2 + oz0 + q) * a.out_plane + ox;
+#pragma unroll
+      for (int o = 0; o < DV_R; ++o) {
+        const int oy = by * DV_TY + ty * DV_R + o;
+        if (oy < a.o1) {
+          if (a.dt_out == IFM_F32) ((float *)a.out)[obase + (int64_t)oy * a.o0] = acc[o];
+          else store_f(a.out, obase + (int64_t)oy * a.o0, a.dt_out, acc[o]);
+        }
+      }
+    }
+    __syncthreads();  // every wave is done with input plane q's slot
+  }
+}
+
+template <int KX, int KY, int KZ>
+static hipError_t launch_small3(const Dense3dArgs &a, const float *taps, dim3 grid, hipStream_t st) {
+  SmallTaps3<KX, KY, KZ> kt;
+  for (int i = 0; i < KX * KY * KZ; ++i) kt.k[i] = taps[i];
+  hipLaunchKernelGGL((k_dense3d_small<KX, KY, KZ>), grid, dim3(256), 0, st, a, kt);
+  return hipGetLastError();
+}
+
+template <int KX, int KY>
+static hipError_t launch_small3_z(int kz, const Dense3dArgs &a, const float *taps, dim3 grid, hipStream_t st) {
+  return kz == 3 ? launch_small3<KX, KY, 3>(a, taps, grid, st) : launch_small3<KX, KY, 5>(a, taps, grid, st);
+}
+
+static bool small3_size(int n) { return n == 3 || n == 5; }
+
+static hipError_t launch_dense3d(const Dense4 &in, Dense4 &out, const int64_t *out_first, const DenseKernel &k,
+                                 int style, float fill, hipStream_t st) {
+  Dense3dArgs a;
+  a.in = in.data;
+  a.out = out.data;
+  a.n0 = (int)in.n[0];
+  a.n1 = (int)in.n[1];
+  a.n2 = (int)in.n[2];
+  a.n3 = (int)in.n[3];
+  a.o0 = (int)out.n[0];
+  a.o1 = (int)out.n[1];
+  a.o2 = (int)out.n[2];
+  a.g0 = (int)out_first[0] + k.first[0];
+  a.g1 = (int)out_first[1] + k.first[1];
+  a.g2 = (int)out_first[2] + k.first[2];
+  a.g3 = (int)out_first[3] + (k.ndim > 3 ? k.first[3] : 0);
+  a.in_plane = in.n[0] * in.n[1];
+  a.out_plane = out.n[0] * out.n[1];
+  a.nzc = (a.o2 + D3_NZ - 1) / D3_NZ;
+  a.style = style;
+  a.dt_in = in.dtype;
+  a.dt_out = out.dtype;
+  a.fill = fill;
+  const dim3 grid((a.o0 + DV_TX - 1) / DV_TX, (a.o1 + DV_TY - 1) / DV_TY, (unsigned)(a.nzc * out.n[3]));
+  const float *taps = k.taps.data();
+  if (k.n[0] == 3) {
+    if (k.n[1] == 3) return launch_small3_z<3, 3>(k.n[2], a, taps, grid, st);
+    return launch_small3_z<3, 5>(k.n[2], a, taps, grid, st);
+  }
+  if (k.n[1] == 3) return launch_small3_z<5, 3>(k.n[2], a, taps, grid, st);
+  return launch_small3_z<5, 5>(k.n[2], a, taps, grid, st);
+}
+
 hipError_t launch_dense_valu(const Dense4 &in, Dense4 &out, const int64_t *out_first, const DenseKernel &k,
                              const float *d_taps, int style, float fill, hipStream_t st) {
   if (out.count() == 0) return hipSuccess;
@@ -323,6 +521,10 @@ hipError_t launch_dense_valu(const Dense4 &in, Dense4 &out, const int64_t *out_f
     hipLaunchKernelGGL(k_dense2d_valu, grid, dim3(256), lds, st, a);
     return hipGetLastError();
   }
+  const bool threed = k.ndim >= 3 && (k.ndim < 4 || k.n[3] == 1) && small3_size(k.n[0]) && small3_size(k.n[1]) &&
+                      small3_size(k.n[2]);
+  if (threed && (out.n[2] + D3_NZ - 1) / D3_NZ * out.n[3] <= 65535)
+    return launch_dense3d(in, out, out_first, k, style, fill, st);
   DenseNdArgs a;
   a.in = in.data;
   a.out = out.data;

@@ -497,6 +497,36 @@ def test_dense_mfma_bf16_storage_and_log():
     assert relerr(out64, want) <= TOL
 
 
+@pytest.mark.parametrize("ks", [(3, 3, 3), (5, 5, 5), (3, 5, 3), (5, 3, 5)])
+def test_dense_3d_register_blocked(ks):
+    """The register-blocked 3-D stencils (3 or 5 taps per axis): volumes with
+    interior and border tiles and several z chunks, kernels off centre, every
+    border, F32 / F64 / UInt8 input, F32 and F64 output, a 4-D image."""
+    rng = np.random.default_rng(500 + 25 * ks[0] + 5 * ks[1] + ks[2])
+    k = rng.standard_normal(ks)
+    first = [-(n // 3) for n in ks]
+    kern = IF.OffsetArray(k, *first)
+    vol = np.asfortranarray(rng.random((150, 70, 45)).astype(np.float32))
+    for b in STYLES + [IF.Fill(0.5)]:
+        out = IF.imfilter(vol, kern, b, IF.Algorithm.FIR())
+        assert out.dtype == np.float64
+        want = O.imfilter_dense(vol.astype(np.float64), k, first, obord(b))
+        assert relerr(out, want) <= TOL, (ks, b)
+    from imagefiltering import _lib
+    out32 = _dense_lib(vol, k, first, IF.Pad("reflect"), _lib.PREC_FP32)
+    assert relerr(out32, O.imfilter_dense(vol.astype(np.float64), k, first, "reflect")) <= TOL
+    v64 = np.asfortranarray(rng.random((70, 33, 12)))
+    assert relerr(IF.imfilter(v64, kern, "circular", IF.Algorithm.FIR()),
+                  O.imfilter_dense(v64, k, first, "circular")) <= TOL
+    codes = np.asfortranarray(rng.integers(0, 256, (66, 40, 9)).astype(np.uint8))
+    assert relerr(IF.imfilter(codes, kern, "symmetric", IF.Algorithm.FIR()),
+                  O.imfilter_dense(codes.astype(np.float64), k, first, "symmetric")) <= TOL
+    v4 = np.asfortranarray(rng.random((60, 40, 20, 3)).astype(np.float32))
+    k4 = IF.OffsetArray(k.reshape(ks + (1,)), *(first + [0]))
+    assert relerr(IF.imfilter(v4, k4, "replicate", IF.Algorithm.FIR()),
+                  O.imfilter_dense(v4.astype(np.float64), k.reshape(ks + (1,)), first + [0], "replicate")) <= TOL
+
+
 def test_dense_3d_box_closed_form():
     img = np.ones((10, 10, 10))
     kern = IF.centered(np.full((3, 3, 3), 1 / 27))

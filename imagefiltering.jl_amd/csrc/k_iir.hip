@@ -293,10 +293,11 @@ constexpr int IS_TP = IS_S + 1;    // transpose buffer pitch (elements): conflic
 //                transpose (rows = (line, half) pairs, lanes along the
 //                segment for the global loads and stores), 16 lines per
 //                round for FP32 input, 8 for FP64 input and for the output.
-constexpr int IS2_L = 32;                 // lines per workgroup
+constexpr int IS_SPW = 2;                 // segments per wave (4, with 16 lines per workgroup, measured slower)
+constexpr int IS2_L = 64 / IS_SPW;        // lines per workgroup
 constexpr int IS2_TB = 16 * IS_TP;        // transpose buffer per wave (doubles)
 __host__ __device__ inline size_t iir_seg2_lds_doubles(int nw, bool rows) {
-  return 6 * IS_S + 2 * (size_t)nw * 3 * IS2_L + (rows ? (size_t)((nw + 1) / 2) * IS2_TB : 0);
+  return 6 * IS_S + 2 * (size_t)nw * 3 * IS2_L + (rows ? (size_t)((nw + IS_SPW - 1) / IS_SPW) * IS2_TB : 0);
 }
 
 struct Seg2 {
@@ -307,8 +308,8 @@ struct Seg2 {
 __device__ __forceinline__ Seg2 seg2_lane(const IirSegTables &tb, int n) {
   Seg2 g;
   const int lane = threadIdx.x & 63;
-  g.ln = lane & (IS2_L - 1);
-  g.sg = 2 * (int)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) + (lane >> 5);
+  g.ln = lane % IS2_L;
+  g.sg = IS_SPW * (int)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) + lane / IS2_L;
   g.has = g.sg < tb.nw;
   g.L = g.has ? min(tb.seg, n - g.sg * tb.seg) : 0;
   g.first = g.sg == 0;
@@ -457,29 +458,29 @@ __global__ void __launch_bounds__(32 * IS_MAXW) k_iir_rows2(IirArgs p, IirSegTab
   const int n = p.n[p.axis];
   const Seg2 g = seg2_lane(tb, n);
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63, half = lane >> 5;
+  const int lane = threadIdx.x & 63, half = lane / IS2_L;
   const int S = tb.seg, nw = tb.nw;
   const int64_t line0 = (int64_t)blockIdx.x * IS2_L;
   double *tr = sm + iir_seg2_lds_doubles(nw, false) + w * IS2_TB;
   const TE *src = (const TE *)p.src;
   double v[IS_S];
   // loads: row i of a round = line r * RI + i / 2, segment 2w + i % 2; lane = element
-  constexpr int RI = DT == IFM_F32 ? 16 : 8;
+  constexpr int RI = (DT == IFM_F32 ? 32 : 16) / IS_SPW;
   TE *tri = (TE *)tr;
   for (int r = 0; r < IS2_L / RI; ++r) {
-    TE x[2 * RI];
+    TE x[IS_SPW * RI];
 #pragma unroll
-    for (int i = 0; i < 2 * RI; ++i) {
-      const int sgi = 2 * w + (i & 1);
+    for (int i = 0; i < IS_SPW * RI; ++i) {
+      const int sgi = IS_SPW * w + i % IS_SPW;
       const int Li = sgi < nw ? min(S, n - sgi * S) : 0;
-      const int64_t ln = line0 + r * RI + (i >> 1);
+      const int64_t ln = line0 + r * RI + i / IS_SPW;
       x[i] = (lane < Li && ln < p.lines) ? src[ln * n + sgi * S + lane] : (TE)0;
     }
 #pragma unroll
-    for (int i = 0; i < 2 * RI; ++i) tri[i * IS_TP + lane] = x[i];
+    for (int i = 0; i < IS_SPW * RI; ++i) tri[i * IS_TP + lane] = x[i];
     __syncthreads();
     if ((g.ln / RI) == r) {
-      const int row = (g.ln - r * RI) * 2 + half;
+      const int row = (g.ln - r * RI) * IS_SPW + half;
 #pragma unroll
       for (int j = 0; j < IS_S; ++j)
         if (j < g.L) v[j] = (double)tri[row * IS_TP + j];
@@ -489,20 +490,20 @@ __global__ void __launch_bounds__(32 * IS_MAXW) k_iir_rows2(IirArgs p, IirSegTab
   seg2_filter(p, tb, g, sm, v);
   const double sc = p.scale;
   // stores: rounds of 8 lines (16 rows of doubles)
-  constexpr int RO = 8;
+  constexpr int RO = 16 / IS_SPW;
   for (int r = 0; r < IS2_L / RO; ++r) {
     if ((g.ln / RO) == r) {
-      const int row = (g.ln - r * RO) * 2 + half;
+      const int row = (g.ln - r * RO) * IS_SPW + half;
 #pragma unroll
       for (int j = 0; j < IS_S; ++j)
         if (j < g.L) tr[row * IS_TP + j] = v[j] * sc;
     }
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < 2 * RO; ++i) {
-      const int sgi = 2 * w + (i & 1);
+    for (int i = 0; i < IS_SPW * RO; ++i) {
+      const int sgi = IS_SPW * w + i % IS_SPW;
       const int Li = sgi < nw ? min(S, n - sgi * S) : 0;
-      const int64_t ln = line0 + r * RO + (i >> 1);
+      const int64_t ln = line0 + r * RO + i / IS_SPW;
       if (lane < Li && ln < p.lines) p.dst[ln * n + sgi * S + lane] = tr[i * IS_TP + lane];
     }
     __syncthreads();
@@ -572,7 +573,7 @@ hipError_t launch_iir(const Dense4 &in, Dense4 &out, int axis, const IirCoef &c,
     // two segments per wave, 32 lines per workgroup
     const bool rows = stride == 1;
     const unsigned nb2 = (unsigned)((p.lines + IS2_L - 1) / IS2_L);
-    const dim3 blk2(64 * ((nw + 1) / 2));
+    const dim3 blk2(64 * ((nw + IS_SPW - 1) / IS_SPW));
     const size_t lds2 = sizeof(double) * iir_seg2_lds_doubles(nw, rows);
     if (rows) {
       if (in.dtype == IFM_F32) hipLaunchKernelGGL((k_iir_rows2<IFM_F32>), dim3(nb2), blk2, lds2, st, p, tb);

@@ -911,6 +911,17 @@ def test_minmax_u8_packed_tiles(w):
         got = IF.mapwindow(IF.extrema, codes, (w, w) + (1,) * len(extra))
         np.testing.assert_array_equal(np.asarray(got[..., 0]), mn.astype(np.uint8), str(shape))
         np.testing.assert_array_equal(np.asarray(got[..., 1]), mx.astype(np.uint8), str(shape))
+    # outputs of a sub-range (unit-step indices) and of the interior (Inner())
+    h = w // 2
+    codes = rng.integers(0, 256, (1024, 40)).astype(np.uint8)
+    c64 = codes.astype(np.float64)
+    inds = (range(5, 205), range(3, 34))
+    got = np.asarray(IF.mapwindow(IF.minimum, codes, (w, w), indices=inds))
+    want = O.minmax(c64, [-h, -h], [w, w], "replicate", [5, 3], [200, 31])[0]
+    np.testing.assert_array_equal(got, want.astype(np.uint8))
+    inner = IF.mapwindow(IF.maximum, codes, (w, w), border=IF.Inner())
+    want = O.minmax(c64, [-h, -h], [w, w], "replicate", [h, h], [1024 - 2 * h, 40 - 2 * h])[1]
+    np.testing.assert_array_equal(np.asarray(inner.data), want.astype(np.uint8))
 
 
 def test_extrema_lemire_semantics():

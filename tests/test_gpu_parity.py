@@ -276,6 +276,13 @@ def test_dense_register_blocked_sizes(ks):
     out = IF.imfilter(IF.Float32, xb, kern, IF.Fill(0.0), IF.Algorithm.FIR())
     want = O.imfilter_dense(IF.bf16_to_float32(xb).astype(np.float64), k, first, ("fill", 0.0))
     assert relerr(out, want) <= TOL
+    # the interior only (Inner()): output (0, 0) is image index -first
+    res = IF.imfilter(img, kern, IF.Inner(), IF.Algorithm.FIR())
+    lo = [-f for f in first]
+    size = [img.shape[d] - ks[d] + 1 for d in range(2)]
+    assert tuple(res.first) == tuple(lo) and res.data.shape == tuple(size)
+    want = O.imfilter_dense(img.astype(np.float64), k, first, "replicate", lo, size)
+    assert relerr(res.data, want) <= TOL
 
 
 def norm_random_kernel(rng, K):
@@ -521,6 +528,11 @@ def test_dense_3d_register_blocked(ks):
     codes = np.asfortranarray(rng.integers(0, 256, (66, 40, 9)).astype(np.uint8))
     assert relerr(IF.imfilter(codes, kern, "symmetric", IF.Algorithm.FIR()),
                   O.imfilter_dense(codes.astype(np.float64), k, first, "symmetric")) <= TOL
+    res = IF.imfilter(vol, kern, IF.Inner(), IF.Algorithm.FIR())
+    lo = [-f for f in first]
+    size = [vol.shape[d] - ks[d] + 1 for d in range(3)]
+    assert tuple(res.first) == tuple(lo) and res.data.shape == tuple(size)
+    assert relerr(res.data, O.imfilter_dense(vol.astype(np.float64), k, first, "replicate", lo, size)) <= TOL
     v4 = np.asfortranarray(rng.random((60, 40, 20, 3)).astype(np.float32))
     k4 = IF.OffsetArray(k.reshape(ks + (1,)), *(first + [0]))
     assert relerr(IF.imfilter(v4, k4, "replicate", IF.Algorithm.FIR()),

@@ -38,13 +38,19 @@ template <> struct MedianNet<7, 2, 2> {
 };
 
 
-// Block = 16 x 16 threads; thread tile TX x TY outputs, PACK tiles stacked in
-// rows (PACK = 2 packs two 16-bit keys per register).
+// Block = TXT x TYT threads (32 x 8, or 16 x 16 where 32 columns of threads
+// would stage more than 128 columns); thread tile TX x TY outputs, PACK tiles
+// stacked in rows (PACK = 2 packs two 16-bit keys per register).
 template <int K, int TX, int TY, int PACK>
 struct NetGeom {
   static constexpr int H = K / 2;
-  static constexpr int BX = 16 * TX;
-  static constexpr int BYH = 16 * TY;  // rows per packed half
+  // threads across x: 32 when the staged width still fits two columns per
+  // lane (wider rows load and store in longer runs: 5x5 on 8192^2 N0f8 189 ->
+  // 172 us), else 16
+  static constexpr int TXT = (32 * TX + K - 1 <= 128) ? 32 : 16;
+  static constexpr int TYT = 256 / TXT;
+  static constexpr int BX = TXT * TX;
+  static constexpr int BYH = TYT * TY;  // rows per packed half
   static constexpr int BY = BYH * PACK;
   static constexpr int LW = BX + K - 1;
   static constexpr int LH = BY + K - 1;
@@ -126,7 +132,7 @@ __device__ __forceinline__ void median_tile(const MedianArgs &a, int bx, int by,
   using G = NetGeom<K, TX, TY, PACK>;
   constexpr int LW = G::LW, LH = G::LH, VW = G::VW, VH = G::VH, BYH = G::BYH;
   const int tid = threadIdx.x;
-  const int tx = tid & 15, ty = tid >> 4;
+  const int tx = tid % G::TXT, ty = tid / G::TXT;
   const int x0 = a.f0 + bx, y0 = a.f1 + by;
   constexpr int PB = PACK == 2 ? BYH : 0;
   switch (a.dt) {
@@ -339,16 +345,17 @@ static uint32_t fill_key32(int dt, double fill) {
 
 template <int K, int TX, int TY, int PACK>
 static void launch_net_on(const MedianArgs &a, hipStream_t st) {
-  constexpr int BX = 16 * TX, BY = 16 * TY * PACK;
-  dim3 grid((a.o0 + BX - 1) / BX, (a.o1 + BY - 1) / BY, a.nplanes < 65535 ? a.nplanes : 65535);
+  using G = NetGeom<K, TX, TY, PACK>;
+  dim3 grid((a.o0 + G::BX - 1) / G::BX, (a.o1 + G::BY - 1) / G::BY, a.nplanes < 65535 ? a.nplanes : 65535);
   hipLaunchKernelGGL((k_median_net<K, TX, TY, PACK>), grid, dim3(256), 0, st, a);
 }
 
 // Batched median: tile geometry of the network kernel for window k.
 void median_batch_tile(int k, bool pack, int *bx, int *by) {
   const int tx = k == 3 ? 4 : 2, ty = 2;
-  *bx = 16 * tx;
-  *by = 16 * ty * (pack ? 2 : 1);
+  const int txt = (32 * tx + k - 1 <= 128) ? 32 : 16;  // NetGeom::TXT
+  *bx = txt * tx;
+  *by = (256 / txt) * ty * (pack ? 2 : 1);
 }
 
 template <int K, int TX, int TY, int PACK>

@@ -70,10 +70,12 @@ struct MfmaGeom {
   static constexpr int MR = (TH + 3) / 4;              // rows per wave in the lane = column part
   static constexpr int HR = (TH + 15) / 16;            // rows per thread in a halo group
   static constexpr int NEL = MR + R16 * HR;            // staged elements per thread
-  // staging batches (register budget); four-chunk kernels take two even
-  // when one would do: their fragments leave too few registers (80 -> 12
-  // bytes of spill, 17 x 17 from 97 to 94 us)
-  static constexpr int NB = ((NEL + 63) / 64 > 1 || NCH < 4) ? (NEL + 63) / 64 : 2;
+  // staging batches (register budget); four-chunk and two-block kernels
+  // take two even when one would do: their fragments leave too few
+  // registers (17 x 17: 80 -> 12 bytes of spill, 97 -> 94 us; 31 x 15:
+  // 204 -> 68 bytes, 121 -> 106 us; 31 x 31 unchanged)
+  static constexpr int NB =
+      ((NEL + 63) / 64 > 1 || (NCH < 4 && KB < 2)) ? (NEL + 63) / 64 : 2;
   static constexpr int JB = (NEL + NB - 1) / NB;       // elements per thread and batch
   static constexpr int RB = 2 * HR;                    // LDS rows per residue class: 8 RB >= 16 HR
   static constexpr int RMAP = 16 * HR;                 // row map entries (every staged row)

@@ -371,7 +371,7 @@ def _dense_lib(img, k, first, border, precision):
     return d_out.to_host()
 
 
-@pytest.mark.parametrize("ks", [(3, 3), (5, 9), (25, 5), (17, 16), (49, 32), (33, 20)])
+@pytest.mark.parametrize("ks", [(3, 3), (5, 9), (25, 5), (17, 16), (49, 32), (33, 20), (31, 15), (40, 12)])
 def test_dense_mfma_explicit_shapes(ks):
     from imagefiltering import _lib
     rng = np.random.default_rng(50 + ks[0] * 7 + ks[1])

@@ -266,8 +266,8 @@ __global__ void __launch_bounds__(256, 2) k_iir_rows(IirArgs p) {
 // full segment, and adds sum_c h_c[j] s_c.  The first segment's forward pass
 // and the last segment's backward pass start from the reference's border
 // initialisation directly (the um / M start of k_iir_strided).
-constexpr int IS_S = 64;     // most elements per segment (doubles in registers per lane)
-constexpr int IS_MAXW = 8;   // most segments: n <= 512
+constexpr int IS_S = 32;     // most elements per segment (doubles in registers per lane)
+constexpr int IS_MAXW = 16;  // most segments: n <= 512
 
 struct IirSegTables {
   double h[3][IS_S];  // forward homogeneous responses: state (y[-1], y[-2], y[-3]) = e_c
@@ -283,16 +283,18 @@ struct IirSegTables {
 constexpr int IS_TP = IS_S + 1;    // transpose buffer pitch (elements): conflict-free both ways
 
 // Two segments per wave: a workgroup owns 32 adjacent lines and lane l of
-// wave w runs segment 2w + (l >> 5) of line l & 31.  A workgroup then needs
-// half the waves of one segment per wave, so two workgroups share a CU (each
-// wave still holds a whole segment in registers) and one's loads and stores
-// overlap the other's recursions.
+// wave w runs segment 2w + (l >> 5) of line l & 31.  Segments of at most 32
+// elements keep a lane's share of the line in 64 registers, so four waves
+// share each SIMD (two workgroups of up to 8 waves per CU) and one's loads
+// and stores overlap the others' recursions; 64-element segments (two waves
+// per SIMD) measured 942 against 767 us on 512 x 512 x 256.
 //  * k_iir_seg2  strided lines: each half-wave reads one contiguous run of 32
 //                lines per element.
-//  * k_iir_rows2 contiguous lines: the segments pass through a per-wave LDS
-//                transpose (rows = (line, half) pairs, lanes along the
-//                segment for the global loads and stores), 16 lines per
-//                round for FP32 input, 8 for FP64 input and for the output.
+//  * k_iir_rows2 contiguous lines: a wave's two segments are one run of up
+//                to 64 elements of a line, loaded and stored by its lanes;
+//                they pass through a per-wave LDS transpose (rows = (line,
+//                half) pairs), 16 lines per round for FP32 input, 8 for FP64
+//                input and for the output.
 constexpr int IS_SPW = 2;                 // segments per wave (4, with 16 lines per workgroup, measured slower)
 constexpr int IS2_L = 64 / IS_SPW;        // lines per workgroup
 constexpr int IS2_TB = 16 * IS_TP;        // transpose buffer per wave (doubles)
@@ -464,20 +466,24 @@ __global__ void __launch_bounds__(32 * IS_MAXW) k_iir_rows2(IirArgs p, IirSegTab
   double *tr = sm + iir_seg2_lds_doubles(nw, false) + w * IS2_TB;
   const TE *src = (const TE *)p.src;
   double v[IS_S];
-  // loads: row i of a round = line r * RI + i / 2, segment 2w + i % 2; lane = element
-  constexpr int RI = (DT == IFM_F32 ? 32 : 16) / IS_SPW;
+  // wave w covers elements [e0, e0 + span) of every line (its two segments);
+  // lane l holds element e0 + l, in transpose row 2 i + l / S, column l % S
+  const int e0 = IS_SPW * w * S, span = min(IS_SPW * S, n - e0);
+  const bool lok = lane < span;
+  const int lh = lane >= S ? 1 : 0, lc = lane - lh * S;
+  constexpr int RI = (DT == IFM_F32 ? 32 : 16) / IS_SPW;  // lines per round
   TE *tri = (TE *)tr;
   for (int r = 0; r < IS2_L / RI; ++r) {
-    TE x[IS_SPW * RI];
+    TE x[RI];
 #pragma unroll
-    for (int i = 0; i < IS_SPW * RI; ++i) {
-      const int sgi = IS_SPW * w + i % IS_SPW;
-      const int Li = sgi < nw ? min(S, n - sgi * S) : 0;
-      const int64_t ln = line0 + r * RI + i / IS_SPW;
-      x[i] = (lane < Li && ln < p.lines) ? src[ln * n + sgi * S + lane] : (TE)0;
+    for (int i = 0; i < RI; ++i) {
+      const int64_t ln = line0 + r * RI + i;
+      x[i] = (lok && ln < p.lines) ? src[ln * n + e0 + lane] : (TE)0;
     }
+    if (lok) {
 #pragma unroll
-    for (int i = 0; i < IS_SPW * RI; ++i) tri[i * IS_TP + lane] = x[i];
+      for (int i = 0; i < RI; ++i) tri[(IS_SPW * i + lh) * IS_TP + lc] = x[i];
+    }
     __syncthreads();
     if ((g.ln / RI) == r) {
       const int row = (g.ln - r * RI) * IS_SPW + half;
@@ -500,11 +506,9 @@ __global__ void __launch_bounds__(32 * IS_MAXW) k_iir_rows2(IirArgs p, IirSegTab
     }
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < IS_SPW * RO; ++i) {
-      const int sgi = IS_SPW * w + i % IS_SPW;
-      const int Li = sgi < nw ? min(S, n - sgi * S) : 0;
-      const int64_t ln = line0 + r * RO + i / IS_SPW;
-      if (lane < Li && ln < p.lines) p.dst[ln * n + sgi * S + lane] = tr[i * IS_TP + lane];
+    for (int i = 0; i < RO; ++i) {
+      const int64_t ln = line0 + r * RO + i;
+      if (lok && ln < p.lines) p.dst[ln * n + e0 + lane] = tr[(IS_SPW * i + lh) * IS_TP + lc];
     }
     __syncthreads();
   }

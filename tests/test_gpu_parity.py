@@ -749,14 +749,16 @@ def test_iir_strided_lines_segments():
     # wave runs one segment (65, 97, 129, 200; last segment shorter), the
     # longest segmented axis (512) and the forms beyond it (513 strided, 600
     # contiguous), partial groups of 32 lines, long and short sigma, F32 / F64
-    # input, contiguous and strided axes
+    # input, contiguous and strided axes; 500 puts waves of full segments
+    # (compile-time bounds) and the wave holding the short last segment in one
+    # workgroup
     rng = np.random.default_rng(29)
     # (every axis is longer than 3: the reference rejects shorter ones)
     for shape, dt, sg in [((4, 65), np.float32, 2.0), ((70, 129), np.float64, 6.0), ((5, 64, 4), np.float32, 1.0),
                           ((4, 513), np.float32, 3.0), ((9, 200, 7), np.float64, 12.0), ((66, 512), np.float32, 2.0),
                           ((4, 4, 130), np.float32, 1.5), ((600, 5), np.float32, 2.0), ((512, 70), np.float64, 2.5),
                           ((33, 97, 5), np.float32, 2.0), ((97, 33), np.float64, 3.0), ((500, 31), np.float32, 4.0),
-                          ((32, 63), np.float32, 1.2)]:
+                          ((32, 63), np.float32, 1.2), ((5, 500), np.float64, 2.0), ((6, 500, 4), np.float32, 3.0)]:
         x = np.asfortranarray(rng.random(shape).astype(dt))
         kern = KernelFactors.IIRGaussian(tuple(sg for _ in shape))
         for border in ["replicate", IF.Fill(0.25)]:
